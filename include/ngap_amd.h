@@ -1,0 +1,151 @@
+/* ngap_amd — MI355X (gfx950) non-blocking automata processing, C ABI.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (NGAP_OK = 0) and leaves a message for ngap_last_error() on failure.
+ * One engine may be used by one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   ngap_frontend_*   load_nfa_from_file + ngap::preprocessing +
+ *                     nfa_utils::group_nfas_by_num + Graph::ReadNFA
+ *                     (code/src/ngap/main.cu:56-78, code/src/ngap/ngap.cu:63-90,
+ *                      code/src/commons/nfa_utils.cpp:382-424,
+ *                      code/src/commons/graph.cu:169-206)
+ *   ngap_load_groups  initGroupCsrWithPrec + GroupMatchset/GroupNodeAttrs/
+ *                     GroupAAS::init (code/src/ngap/ngap.cu:3086-3111,
+ *                     code/include/commons/group_graph.h:35-109)
+ *   ngap_set_input    concat_input_streams_to_array2 + copy_to_device
+ *                     (code/src/gpunfautils/abstract_gpunfa.cu:55-81,
+ *                      code/src/ngap/ngap.cu:1482-1483)
+ *   ngap_run          NonBlockingBuffer::init_nfagroups + the kernel launch in
+ *                     ngap::launch_non_blocking_*_groups (code/src/ngap/
+ *                     ngap_buffer.cu:475-748, code/src/ngap/ngap.cu:284-1990;
+ *                     kernels code/src/ngap/kernel.h:9-55)
+ *   ngap_get_reports  D2H + pack + sort by (offset, id) + unique
+ *                     (code/src/ngap/ngap.cu:1899-1933, compareResult :19-30)
+ */
+#ifndef NGAP_AMD_H_
+#define NGAP_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define NGAP_OK 0
+#define NGAP_ERR_INVALID 1     /* bad argument or malformed input */
+#define NGAP_ERR_DEVICE 2      /* HIP runtime error / no device */
+#define NGAP_ERR_OOM 3         /* device allocation failed */
+#define NGAP_ERR_OVERFLOW 4    /* worklist or report capacity exceeded */
+#define NGAP_ERR_STATE 5       /* call out of order (no groups / no input) */
+
+/* --algorithm names (code/src/ngap/main.cu:114-130) */
+#define NGAP_ALGO_BLOCKING 0        /* blockinggroups */
+#define NGAP_ALGO_NAP 1             /* NAPgroups */
+#define NGAP_ALGO_NONBLOCKING 2     /* nonblockinggroups (O1) */
+#define NGAP_ALGO_R1 3              /* nonblockingr1groups (O4, one level) */
+#define NGAP_ALGO_R2 4              /* nonblockingr2groups (O4, two levels) */
+#define NGAP_ALGO_PC 5              /* nonblockingpcgroups (O1+O2) */
+#define NGAP_ALGO_ALL 6             /* nonblockingallgroups (O1+O2+O3) */
+
+/* scheduling of (group, chunk) work */
+#define NGAP_SCHED_PLAIN 0     /* one workgroup per (group, chunk) unit */
+#define NGAP_SCHED_BATCHED 1   /* persistent waves over a device work queue */
+
+typedef struct ngap_engine ngap_engine;
+
+/* One group as Graph/Csr hold it (code/include/commons/graph.h:101-248). */
+typedef struct {
+  int32_t num_states;
+  const int32_t *row_offsets;   /* [num_states+1], edges sorted by (src,dst) */
+  const int32_t *col_indices;   /* [row_offsets[num_states]] */
+  const uint32_t *symbol_sets;  /* [num_states*8], bit b in word b/32 */
+  const uint8_t *node_attrs;    /* bit0 all-input start, bit1 reporting */
+  int32_t num_always_active;
+  const int32_t *always_active; /* all-input states */
+  int32_t num_start;
+  const int32_t *start_states;  /* start-of-data states (not all-input) */
+} ngap_group_desc;
+
+/* Per-run options.  Names follow the reference flags
+ * (code/src/ngap/ngap_option.h:12-68, code/include/commons/common_func.h:50-115). */
+typedef struct {
+  int32_t algorithm;            /* NGAP_ALGO_* */
+  int32_t schedule;             /* NGAP_SCHED_* */
+  int32_t precompute_depth;     /* --precompute-depth, 0..3 */
+  int32_t active_threshold;     /* --active-threshold */
+  int32_t unique;               /* --unique: de-duplicate (state, offset) items */
+  int32_t is_dup;               /* start-of-data states in every chunk */
+  int32_t report_off;           /* --report-off: count reports only */
+  uint32_t data_buffer_fetch_size; /* --data-buffer-fetch-size (tuning hint) */
+  uint32_t add_aas_start;       /* --add-aan-start (tuning hint) */
+  uint32_t add_aas_interval;    /* --add-aas-interval: root batch positions */
+  uint64_t result_capacity;     /* --result-capacity (reports) */
+  uint32_t window;              /* batched: positions per work item, 0 = auto */
+  uint32_t spill_capacity;      /* items per wave beyond the LDS ring, 0 = auto */
+  uint32_t persistent_blocks;   /* batched: workgroups, 0 = auto */
+  int32_t use_graph;            /* batched: replay a captured hipGraph */
+  const uint32_t *task_order;   /* batched: optional work-item order (host) */
+  uint32_t task_order_len;
+} ngap_run_opts;
+
+typedef struct {
+  double kernel_ms;             /* device time of the matching launch(es) */
+  uint64_t reports_raw;         /* reports written (before sort/unique) */
+  uint64_t overflow_units;      /* units/work items whose worklist overflowed */
+  uint64_t items;               /* (state, offset) items expanded */
+  uint32_t num_units;           /* (group, chunk) pairs */
+  uint32_t num_tasks;           /* batched work items */
+} ngap_run_stats;
+
+const char *ngap_last_error(void);
+void ngap_default_opts(ngap_run_opts *opts);
+
+int ngap_create(int32_t device, ngap_engine **out);
+int ngap_destroy(ngap_engine *eng);
+
+/* Builds the device automata image (memo tables up to max_precompute_depth). */
+int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups,
+                     int32_t num_groups, int32_t max_precompute_depth);
+
+/* Uploads num_seg chunks of chunk_len bytes (the concatenated input). */
+int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
+                   uint32_t num_seg);
+
+int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats);
+
+/* Sorted, de-duplicated reports of the last run, reference encoding
+ * ((state | group << 22) << 32) | offset.  *n_out gets the total count; at
+ * most cap entries are written (out may be NULL to query the count). */
+int ngap_get_reports(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out);
+
+/* Per-work-item item counts of the last batched run (frontier traces). */
+int ngap_get_task_items(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out);
+
+/* ---- front-end: ANML -> groups in the reference state-id namespace ---- */
+typedef struct ngap_frontend ngap_frontend;
+
+/* group_num: --group-num (lowered to the component count when larger),
+ * max_nfa_size: --max-nfa-size (-1 off), quit_degree / remove_degree as the
+ * reference flags. */
+int ngap_frontend_load(const char *anml_path, int32_t group_num, int32_t max_nfa_size,
+                       int32_t quit_degree, int32_t remove_degree, ngap_frontend **out);
+int ngap_frontend_load_text(const char *anml_text, int32_t group_num,
+                            int32_t max_nfa_size, ngap_frontend **out);
+int ngap_frontend_free(ngap_frontend *fe);
+int32_t ngap_frontend_num_groups(const ngap_frontend *fe);
+/* Fills desc with pointers owned by fe (valid until ngap_frontend_free). */
+int ngap_frontend_group(const ngap_frontend *fe, int32_t g, ngap_group_desc *desc);
+/* Name of state s of group g (ANML id, with _dup_ suffixes after splits). */
+const char *ngap_frontend_state_name(const ngap_frontend *fe, int32_t g, int32_t s);
+/* Text dump of all groups (same format as the reference-side dumper). */
+const char *ngap_frontend_dump(ngap_frontend *fe);
+/* Parses one VASim symbol-set string into 8 words. */
+int ngap_parse_symbol_set(const char *s, uint32_t words_out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGAP_AMD_H_ */

@@ -1,0 +1,129 @@
+"""Loader for libngap_amd.so (the C ABI declared in include/ngap_amd.h).
+
+The library is built in-tree by ``make -C ngap_amd`` (``__graft_entry__.build``).
+There is no fallback: a missing or stale library raises immediately.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libngap_amd.so")
+
+NGAP_OK = 0
+NGAP_ERR_INVALID = 1
+NGAP_ERR_DEVICE = 2
+NGAP_ERR_OOM = 3
+NGAP_ERR_OVERFLOW = 4
+NGAP_ERR_STATE = 5
+
+ALGORITHMS = {  # --algorithm names (code/src/ngap/main.cu:114-130)
+    "blockinggroups": 0,
+    "NAPgroups": 1,
+    "nonblockinggroups": 2,
+    "nonblockingr1groups": 3,
+    "nonblockingr2groups": 4,
+    "nonblockingpcgroups": 5,
+    "nonblockingallgroups": 6,
+}
+SCHED_PLAIN = 0
+SCHED_BATCHED = 1
+
+
+class GroupDesc(C.Structure):
+    _fields_ = [
+        ("num_states", C.c_int32),
+        ("row_offsets", C.c_void_p),
+        ("col_indices", C.c_void_p),
+        ("symbol_sets", C.c_void_p),
+        ("node_attrs", C.c_void_p),
+        ("num_always_active", C.c_int32),
+        ("always_active", C.c_void_p),
+        ("num_start", C.c_int32),
+        ("start_states", C.c_void_p),
+    ]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [
+        ("algorithm", C.c_int32),
+        ("schedule", C.c_int32),
+        ("precompute_depth", C.c_int32),
+        ("active_threshold", C.c_int32),
+        ("unique", C.c_int32),
+        ("is_dup", C.c_int32),
+        ("report_off", C.c_int32),
+        ("data_buffer_fetch_size", C.c_uint32),
+        ("add_aas_start", C.c_uint32),
+        ("add_aas_interval", C.c_uint32),
+        ("result_capacity", C.c_uint64),
+        ("window", C.c_uint32),
+        ("spill_capacity", C.c_uint32),
+        ("persistent_blocks", C.c_uint32),
+        ("use_graph", C.c_int32),
+        ("task_order", C.c_void_p),
+        ("task_order_len", C.c_uint32),
+    ]
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("reports_raw", C.c_uint64),
+        ("overflow_units", C.c_uint64),
+        ("items", C.c_uint64),
+        ("num_units", C.c_uint32),
+        ("num_tasks", C.c_uint32),
+    ]
+
+
+# Every symbol include/ngap_amd.h declares, with (restype, argtypes).
+SIGNATURES = {
+    "ngap_last_error": (C.c_char_p, []),
+    "ngap_default_opts": (None, [C.POINTER(RunOpts)]),
+    "ngap_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "ngap_destroy": (C.c_int, [C.c_void_p]),
+    "ngap_load_groups": (C.c_int, [C.c_void_p, C.POINTER(GroupDesc), C.c_int32, C.c_int32]),
+    "ngap_set_input": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32]),
+    "ngap_run": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
+    "ngap_get_reports": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "ngap_get_task_items": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "ngap_frontend_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     C.POINTER(C.c_void_p)]),
+    "ngap_frontend_load_text": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "ngap_frontend_free": (C.c_int, [C.c_void_p]),
+    "ngap_frontend_num_groups": (C.c_int32, [C.c_void_p]),
+    "ngap_frontend_group": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(GroupDesc)]),
+    "ngap_frontend_state_name": (C.c_char_p, [C.c_void_p, C.c_int32, C.c_int32]),
+    "ngap_frontend_dump": (C.c_char_p, [C.c_void_p]),
+    "ngap_parse_symbol_set": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32 * 8)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Returns the loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C ngap_amd` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class NgapError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ngap error {code}: {msg}")
+        self.code = code
+
+
+def check(code):
+    if code != NGAP_OK:
+        raise NgapError(code, lib().ngap_last_error().decode(errors="replace"))

@@ -1,0 +1,70 @@
+// Device-side structures shared by the HIP kernels and the host runtime.
+#pragma once
+
+#include <cstdint>
+
+#include "layout.h"
+
+#if defined(__HIPCC__)
+#define NGAP_HD __host__ __device__
+#else
+#define NGAP_HD
+#endif
+
+namespace ngapamd {
+
+struct DevImage {
+  const GroupMeta *meta;
+  const int32_t *succ4;      // 4 per record (read as int4)
+  const uint32_t *cls;       // 8 per state
+  const int32_t *sod;
+  const uint32_t *beg[kMaxMemoDepth];
+  const uint32_t *nrep[kMaxMemoDepth];
+  const int32_t *list[kMaxMemoDepth];
+  const uint32_t *key[kMaxMemoDepth];
+  const uint32_t *dir[kMaxMemoDepth];
+};
+
+// One batched work item: roots at positions [begin, end) of chunk `chunk` of
+// group `group`; descendants run to the end of the chunk.
+struct Task {
+  uint32_t group, chunk, begin, end;
+};
+
+struct KParams {
+  DevImage img;
+  const uint8_t *input;
+  uint32_t chunk_len, num_seg, num_groups;
+  int32_t memo_depth;      // 1..3, d-gram roots
+  int32_t is_dup;
+  int32_t dedup;
+  int32_t priv_limit;      // in-lane continuation steps, -1 = unlimited
+  int32_t report_off;
+  uint32_t ring_cap;       // LDS ring entries per wave (power of two)
+  uint32_t rbuf_cap;       // LDS report buffer entries per wave
+  uint32_t dc_cap;         // LDS dedup cache entries per wave (power of two)
+  uint32_t root_batch;     // plain: positions claimed per cursor step
+  unsigned long long *report_count;
+  uint64_t *reports;
+  uint64_t report_cap;
+  uint64_t *spill;
+  uint32_t spill_cap;      // items per wave
+  uint32_t *overflow;      // plain: per unit; batched: per task
+  unsigned long long *items;   // expanded items (statistic)
+  const Task *tasks;
+  const uint32_t *task_order;  // nullable
+  uint32_t num_tasks;
+  uint32_t *task_head;
+  unsigned long long *task_items;  // nullable, per task
+  uint32_t waves_per_block;
+};
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kBlockThreads = 256;
+
+// LDS bytes per wave for the given capacities
+NGAP_HD inline uint32_t wave_lds_bytes(uint32_t ring, uint32_t rbuf, uint32_t dc) {
+  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64) * 4 + 4;
+}
+
+}  // namespace ngapamd

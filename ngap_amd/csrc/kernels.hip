@@ -1,0 +1,499 @@
+// HIP kernels for gfx950 (wave64): the non-blocking (state, offset) worklist
+// engine.
+//
+// Reference behaviour (what is computed): the worklist kernels
+// advanceAndFilterNonBlocking{NAP,,R1,R2,Prec,All}Groups
+// (code/src/ngap/kernel_bap.cu:10-185, kernel_ngap_O1.cu:10-185,
+// kernel_ngap_O4.cu:9-548, kernel_ngap_O3.cu:10-295, kernel_ngap_OA.cu:10-374):
+// an item (v, p) means state v matched input[p]; expanding it tests every
+// successor u of v against input[p+1], reports u when it reports, and enables
+// (u, p+1).  All-input states are enabled at every position; start-of-data
+// states at the first position of chunk 0 (or of every chunk when the input
+// was duplicated).  The report set is the reference's; item order is free.
+//
+// How it is computed here (MI355X-first, not the reference's structure):
+//  * Each wave owns a ring of items in LDS.  Pushes are compacted with
+//    ballot + mbcnt (one ring slot computation per wave, no atomics at all:
+//    head/tail are wave-uniform registers).  Items beyond the ring go to a
+//    per-wave HBM spill area and come back when the ring runs low.
+//  * Idle lanes refill from the ring every step, so a wave stays converged
+//    and busy without block barriers.
+//  * A lane keeps the first surviving successor of its item in registers and
+//    continues with it (work privatization generalised to whole chains; the
+//    reference expands one more level, kernel_ngap_OA.cu:96-138).
+//  * Roots come from the memo tables: one lookup per position gives the
+//    states matched d-1 positions later (prefix memoization,
+//    kernel_ngap_OA.cu:243-310), flattened across the wave with a prefix sum.
+//  * Reports are compacted into a per-wave LDS buffer and flushed with one
+//    global atomic per flush (the reference: one atomic per report,
+//    kernel_helper.h:78-104).
+//  * (state, offset) items of states with two or more incoming edges pass a
+//    lossy LDS cache (64-bit atomic exchange): an exact duplicate seen
+//    recently is dropped.  A duplicate is never needed (set semantics), and a
+//    first occurrence is never dropped.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ngapamd {
+
+#define NGAP_DEV __device__ __forceinline__
+
+NGAP_DEV uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+NGAP_DEV uint32_t mask_rank(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+NGAP_DEV uint64_t wballot(bool p) { return __ballot(p); }
+NGAP_DEV void wsync() { __builtin_amdgcn_wave_barrier(); }
+
+NGAP_DEV uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  return x;
+}
+
+struct WaveState {
+  uint64_t *ring;
+  uint64_t *rbuf;
+  uint64_t *dc;
+  uint32_t *sh_pref;  // 65
+  uint32_t *sh_beg;   // 64
+  uint32_t *sh_pos;   // 64
+  uint64_t *spill;
+  uint32_t head, tail, rn, spill_n;
+  uint32_t lane;
+  bool overflow;
+  unsigned long long items;
+  // current (group, chunk)
+  uint32_t g, gtag, rec_base, cls_base, cstart, cend;
+  uint32_t key_base2, key_base3;
+};
+
+NGAP_DEV bool class_has(const KParams &P, const WaveState &W, uint32_t u, uint32_t b) {
+  return (P.img.cls[(size_t)(W.cls_base + u) * 8 + (b >> 5)] >> (b & 31)) & 1u;
+}
+
+NGAP_DEV uint64_t item_of(uint32_t u, uint32_t pos) { return ((uint64_t)pos << 32) | u; }
+
+NGAP_DEV void push(const KParams &P, WaveState &W, bool pred, uint64_t item) {
+  uint64_t m = wballot(pred);
+  if (m == 0) return;
+  uint32_t cnt = __popcll(m);
+  uint32_t r = mask_rank(m);
+  if (W.tail - W.head + cnt <= P.ring_cap) {
+    if (pred) W.ring[(W.tail + r) & (P.ring_cap - 1)] = item;
+    W.tail += cnt;
+  } else if (W.spill_n + cnt <= P.spill_cap) {
+    if (pred) W.spill[W.spill_n + r] = item;
+    W.spill_n += cnt;
+  } else {
+    W.overflow = true;
+  }
+  wsync();
+}
+
+NGAP_DEV void flush_reports(const KParams &P, WaveState &W) {
+  if (W.rn == 0) return;
+  wsync();
+  unsigned long long base = 0;
+  if (W.lane == 0) base = atomicAdd(P.report_count, (unsigned long long)W.rn);
+  base = __shfl(base, 0, 64);
+  if (!P.report_off) {
+    for (uint32_t i = W.lane; i < W.rn; i += kWave) {
+      unsigned long long idx = base + i;
+      if (idx < P.report_cap) P.reports[idx] = W.rbuf[i];
+    }
+  }
+  W.rn = 0;
+  wsync();
+}
+
+// Report encoding on the device: (offset << 32) | (state | group << 22), so a
+// plain numeric sort gives the reference's (offset, id) order.
+NGAP_DEV void emit(const KParams &P, WaveState &W, bool pred, uint64_t val) {
+  uint64_t m = wballot(pred);
+  if (m == 0) return;
+  uint32_t cnt = __popcll(m);
+  if (W.rn + cnt > P.rbuf_cap) flush_reports(P, W);
+  if (pred) W.rbuf[W.rn + mask_rank(m)] = val;
+  W.rn += cnt;
+  wsync();
+}
+
+NGAP_DEV bool dedup_fresh(const KParams &P, WaveState &W, uint64_t key) {
+  uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (P.dc_cap - 1);
+  unsigned long long old = atomicExch((unsigned long long *)&W.dc[h], (unsigned long long)key);
+  return old != key;
+}
+
+// Spilled items return to the ring when it runs low.
+NGAP_DEV void refill_from_spill(const KParams &P, WaveState &W) {
+  uint32_t room = P.ring_cap - (W.tail - W.head);
+  uint32_t k = W.spill_n < room ? W.spill_n : room;
+  uint32_t from = W.spill_n - k;
+  for (uint32_t i = W.lane; i < k; i += kWave)
+    W.ring[(W.tail + i) & (P.ring_cap - 1)] = W.spill[from + i];
+  W.tail += k;
+  W.spill_n = from;
+  wsync();
+}
+
+// Processes a variable number of list entries per lane as one flat index
+// space (prefix sum over the wave), 64 entries per step.
+// MODE 0: emit each entry as a report; MODE 1: push each entry as an item.
+template <int MODE>
+NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg,
+                      uint32_t pos, const int32_t *list) {
+  uint32_t incl = wave_incl_scan(cnt, W.lane);
+  uint32_t total = __shfl(incl, 63, 64);
+  if (total == 0) return;
+  W.sh_pref[W.lane] = incl - cnt;
+  W.sh_beg[W.lane] = beg;
+  W.sh_pos[W.lane] = pos;
+  if (W.lane == 63) W.sh_pref[64] = total;
+  wsync();
+  for (uint32_t base = 0; base < total; base += kWave) {
+    uint32_t j = base + W.lane;
+    bool ok = j < total;
+    int32_t e = 0;
+    uint32_t ppos = 0;
+    if (ok) {
+      int lo = 0, hi = 64;  // largest o with pref[o] <= j (pref[64] = total > j)
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (W.sh_pref[mid] <= j) lo = mid;
+        else hi = mid - 1;
+      }
+      e = list[W.sh_beg[lo] + (j - W.sh_pref[lo])];
+      ppos = W.sh_pos[lo];
+    }
+    uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+    if (MODE == 0) {
+      emit(P, W, ok, ((uint64_t)ppos << 32) | u | W.gtag);
+    } else {
+      bool keep = ok && !(e & kFlagDeadEnd);
+      if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, ppos));
+      push(P, W, keep, item_of(u, ppos));
+    }
+  }
+  wsync();
+}
+
+NGAP_DEV int32_t memo_lookup(const KParams &P, const WaveState &W, int d, uint32_t p) {
+  const uint8_t *in = P.input;
+  if (d == 1) return (int32_t)(W.g * 256u + in[p]);
+  uint32_t key = 0;
+  for (int i = 0; i < d; i++) key = (key << 8) | in[p + i];
+  uint32_t first = key >> (8 * (d - 1));
+  const uint32_t *dir = P.img.dir[d - 1] + (size_t)W.g * 257;
+  uint32_t kb = d == 2 ? W.key_base2 : W.key_base3;
+  int lo = (int)(kb + dir[first]), hi = (int)(kb + dir[first + 1]) - 1;
+  const uint32_t *keys = P.img.key[d - 1];
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    uint32_t k = keys[mid];
+    if (k == key) return mid;
+    if (k < key) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+// Roots for positions [p0, pend): one lane per position.  A lane uses the
+// deepest memo level that fits before the chunk end (kernel_ngap_OA.cu:243-
+// 244); shallower levels contribute their reports only.
+NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
+  uint32_t p = p0 + W.lane;
+  bool act = p < pend;
+  uint32_t left = act ? W.cend - p : 0;
+  uint32_t k = act ? (left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth) : 0;
+  for (int d = 1; d <= P.memo_depth; d++) {
+    bool use = act && (uint32_t)d <= k;
+    uint32_t beg = 0, len = 0, nr = 0;
+    if (use) {
+      int32_t li = memo_lookup(P, W, d, p);
+      if (li >= 0) {
+        beg = P.img.beg[d - 1][li];
+        len = P.img.beg[d - 1][li + 1] - beg;
+        nr = P.img.nrep[d - 1][li];
+      }
+    }
+    flatten<0>(P, W, nr, beg, p + d - 1, P.img.list[d - 1]);
+    bool items = use && (uint32_t)d == k && (p + d < W.cend);
+    flatten<1>(P, W, items ? len : 0, beg, p + d - 1, P.img.list[d - 1]);
+  }
+}
+
+NGAP_DEV void seed_start_of_data(const KParams &P, WaveState &W, uint32_t sod_begin,
+                                 uint32_t sod_end) {
+  uint32_t b = P.input[W.cstart];
+  for (uint32_t i = sod_begin; i < sod_end; i += kWave) {
+    uint32_t idx = i + W.lane;
+    bool ok = idx < sod_end;
+    int32_t e = ok ? P.img.sod[idx] : 0;
+    uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+    bool hit = ok && class_has(P, W, u, b);
+    emit(P, W, hit && (e & kFlagReport), ((uint64_t)W.cstart << 32) | u | W.gtag);
+    bool keep = hit && !(e & kFlagDeadEnd) && (W.cstart + 1 < W.cend);
+    if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, W.cstart));
+    push(P, W, keep, item_of(u, W.cstart));
+  }
+}
+
+// One step of every valid lane's item: test its successors on the next byte.
+NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
+                     uint32_t &p, int &depth) {
+  const uint32_t p1 = p + 1;
+  uint32_t b = 0;
+  int4 rec = make_int4(-1, -1, -1, -1);
+  if (valid) {
+    b = P.input[p1];
+    rec = reinterpret_cast<const int4 *>(P.img.succ4)[W.rec_base + v];
+  }
+  W.items += __popcll(wballot(valid));
+  bool has_next = false;
+  uint32_t nv = 0;
+  bool more = valid;
+  const bool can_priv = P.priv_limit < 0 || depth < P.priv_limit;
+  while (wballot(more)) {
+    int32_t link = -1;
+    int32_t es[4] = {rec.x, rec.y, rec.z, rec.w};
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int32_t e = es[s];
+      const bool edge = more && e >= 0;
+      if (more && e <= -2) link = -(e + 2);
+      const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+      bool hit = false;
+      if (edge) hit = class_has(P, W, u, b);
+      emit(P, W, hit && (e & kFlagReport), ((uint64_t)p1 << 32) | u | W.gtag);
+      bool keep = hit && !(e & kFlagDeadEnd) && (p1 + 1 < W.cend);
+      if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, p1));
+      const bool take = keep && !has_next && can_priv;
+      if (take) {
+        has_next = true;
+        nv = u;
+      }
+      push(P, W, keep && !take, item_of(u, p1));
+    }
+    more = more && link >= 0;
+    if (more) rec = reinterpret_cast<const int4 *>(P.img.succ4)[link];
+  }
+  if (has_next) {
+    v = nv;
+    p = p1;
+    depth++;
+  } else {
+    valid = false;
+  }
+}
+
+NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t c) {
+  const GroupMeta gm = P.img.meta[g];
+  W.g = g;
+  W.gtag = g << 22;
+  W.rec_base = gm.rec_base;
+  W.cls_base = gm.cls_base;
+  W.key_base2 = gm.memo_key_base[0];
+  W.key_base3 = gm.memo_key_base[1];
+  W.cstart = c * P.chunk_len;
+  W.cend = W.cstart + P.chunk_len;
+  for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
+  wsync();
+}
+
+NGAP_DEV void setup_wave(const KParams &P, WaveState &W, unsigned char *base,
+                         uint64_t *spill) {
+  W.ring = reinterpret_cast<uint64_t *>(base);
+  W.rbuf = W.ring + P.ring_cap;
+  W.dc = W.rbuf + P.rbuf_cap;
+  W.sh_pref = reinterpret_cast<uint32_t *>(W.dc + P.dc_cap);
+  W.sh_beg = W.sh_pref + 65;
+  W.sh_pos = W.sh_beg + 64;
+  W.spill = spill;
+  W.head = W.tail = W.rn = W.spill_n = 0;
+  W.lane = lane_id();
+  W.overflow = false;
+  W.items = 0;
+}
+
+// Runs items to completion, pulling root batches from `src` whenever the
+// ring runs low.  Every loop iteration is wave-uniform.
+template <class Src>
+NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
+  bool valid = false;
+  uint32_t v = 0, p = 0;
+  int depth = 0;
+  bool roots_left = true;
+  for (;;) {
+    uint32_t qn = W.tail - W.head;
+    if (qn < kWave && W.spill_n > 0) {
+      refill_from_spill(P, W);
+      qn = W.tail - W.head;
+    }
+    if (qn < kWave && W.spill_n == 0 && roots_left) {
+      uint32_t a, b;
+      if (src.next(W, a, b)) roots(P, W, a, b);
+      else roots_left = false;
+      continue;
+    }
+    const uint64_t idle = wballot(!valid);
+    uint32_t avail = W.tail - W.head;
+    uint32_t nidle = __popcll(idle);
+    uint32_t take = nidle < avail ? nidle : avail;
+    uint32_t r = mask_rank(idle);
+    if (!valid && r < take) {
+      uint64_t it = W.ring[(W.head + r) & (P.ring_cap - 1)];
+      v = (uint32_t)it;
+      p = (uint32_t)(it >> 32);
+      valid = true;
+      depth = 0;
+    }
+    W.head += take;
+    wsync();
+    if (wballot(valid) == 0) {
+      if (W.tail == W.head && W.spill_n == 0 && !roots_left) break;
+      continue;
+    }
+    expand(P, W, valid, v, p, depth);
+  }
+}
+
+// Plain schedule: the cursor over a (group, chunk) unit shared by the block's
+// waves (the reference's one-block-per-unit decomposition, ngap.cu:1507).
+struct SharedCursor {
+  uint32_t *cursor;
+  uint32_t end, batch;
+  NGAP_DEV bool next(const WaveState &W, uint32_t &a, uint32_t &b) {
+    uint32_t x = 0;
+    if (W.lane == 0) x = atomicAdd(cursor, batch);
+    x = __shfl(x, 0, 64);
+    if (x >= end) return false;
+    a = x;
+    b = x + batch < end ? x + batch : end;
+    return true;
+  }
+};
+
+// Batched schedule: a wave walks its own work item in 64-position steps.
+struct LocalCursor {
+  uint32_t cur, end;
+  NGAP_DEV bool next(const WaveState &W, uint32_t &a, uint32_t &b) {
+    if (cur >= end) return false;
+    a = cur;
+    b = cur + kWave < end ? cur + kWave : end;
+    cur = b;
+    return true;
+  }
+};
+
+__global__ void __launch_bounds__(kBlockThreads) k_plain(KParams P) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t unit = blockIdx.x;
+  const uint32_t g = unit % P.num_groups, c = unit / P.num_groups;
+  uint32_t *cursor = reinterpret_cast<uint32_t *>(lds);
+  const uint32_t wbytes = wave_lds_bytes(P.ring_cap, P.rbuf_cap, P.dc_cap);
+  WaveState W;
+  setup_wave(P, W, lds + 16 + wave * wbytes,
+             P.spill + ((size_t)unit * P.waves_per_block + wave) * P.spill_cap);
+  begin_unit(P, W, g, c);
+  if (threadIdx.x == 0) *cursor = W.cstart;
+  __syncthreads();
+  if (wave == 0 && (c == 0 || P.is_dup)) {
+    const GroupMeta gm = P.img.meta[g];
+    seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
+  }
+  SharedCursor src{cursor, W.cend, P.root_batch};
+  drain(P, W, src);
+  flush_reports(P, W);
+  if (W.lane == 0) {
+    if (W.overflow) atomicOr(&P.overflow[unit], 1u);
+    atomicAdd(P.items, W.items);
+  }
+}
+
+__global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t wbytes = wave_lds_bytes(P.ring_cap, P.rbuf_cap, P.dc_cap);
+  WaveState W;
+  setup_wave(P, W, lds + 16 + wave * wbytes,
+             P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
+  for (;;) {
+    uint32_t t = 0;
+    if (W.lane == 0) t = atomicAdd(P.task_head, 1u);
+    t = __shfl(t, 0, 64);
+    if (t >= P.num_tasks) break;
+    const uint32_t ti = P.task_order ? P.task_order[t] : t;
+    const Task T = P.tasks[ti];
+    begin_unit(P, W, T.group, T.chunk);
+    W.overflow = false;
+    const unsigned long long items0 = W.items;
+    if (T.begin == W.cstart && (T.chunk == 0 || P.is_dup)) {
+      const GroupMeta gm = P.img.meta[T.group];
+      seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
+    }
+    LocalCursor src{T.begin, T.end};
+    drain(P, W, src);
+    if (W.lane == 0) {
+      if (W.overflow) P.overflow[ti] = 1u;
+      if (P.task_items) P.task_items[ti] = W.items - items0;
+    }
+  }
+  flush_reports(P, W);
+  if (W.lane == 0) atomicAdd(P.items, W.items);
+}
+
+// Clears the per-run counters with a kernel (graph-capture friendly).
+__global__ void k_reset(unsigned long long *report_count, unsigned long long *items,
+                        uint32_t *task_head, uint32_t *overflow, uint32_t n_overflow) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *report_count = 0;
+    *items = 0;
+    *task_head = 0;
+  }
+  for (; i < n_overflow; i += gridDim.x * blockDim.x) overflow[i] = 0;
+}
+
+// Host-side launch wrappers (this translation unit owns the kernels).
+hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
+                        uint32_t *head, uint32_t *ovf, uint32_t n) {
+  uint32_t blocks = (n + 255) / 256;
+  if (blocks == 0) blocks = 1;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(256), 0, s, rc, items, head, ovf, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds) {
+  hipLaunchKernelGGL(k_plain, dim3(units), dim3(kBlockThreads), lds, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds) {
+  hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+  return hipGetLastError();
+}
+
+hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batched,
+                                                      kBlockThreads, lds);
+}
+
+hipError_t set_lds_limit(uint32_t lds) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_plain),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+}  // namespace ngapamd

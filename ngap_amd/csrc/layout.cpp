@@ -1,0 +1,216 @@
+// Device image builder (host side).  Layout rationale in layout.h.
+#include "layout.h"
+
+#include <algorithm>
+#include <bitset>
+#include <sstream>
+
+namespace ngapamd {
+
+uint64_t Image::bytes() const {
+  uint64_t b = meta.size() * sizeof(GroupMeta) + succ4.size() * 4 + cls.size() * 4 +
+               sod.size() * 4;
+  for (auto &m : memo)
+    b += (m.key.size() + m.beg.size() + m.nrep.size() + m.dir.size()) * 4 +
+         m.list.size() * 4;
+  return b;
+}
+
+namespace {
+
+inline bool has_byte(const uint32_t *sets, int v, int b) {
+  return (sets[(size_t)v * 8 + (b >> 5)] >> (b & 31)) & 1u;
+}
+
+// Appends one list (reporting entries first, each part in id order).
+void push_list(MemoLevel &m, std::vector<int32_t> &ids, const std::vector<int32_t> &tag) {
+  std::sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) {
+    bool ra = tag[a] & kFlagReport, rb = tag[b] & kFlagReport;
+    if (ra != rb) return ra;
+    return a < b;
+  });
+  uint32_t nrep = 0;
+  for (int32_t v : ids) {
+    if (tag[v] & kFlagReport) nrep++;
+    m.list.push_back(tag[v]);
+  }
+  m.nrep.push_back(nrep);
+  m.beg.push_back((uint32_t)m.list.size());
+}
+
+}  // namespace
+
+bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth,
+                 Image &img, std::string &err) {
+  img = Image();
+  if (num_groups <= 0 || num_groups > 1024) {
+    err = "number of groups must be in [1, 1024] (10 group bits in report ids)";
+    return false;
+  }
+  if (memo_depth < 1) memo_depth = 1;
+  if (memo_depth > kMaxMemoDepth) {
+    err = "precompute depth must be <= 3";
+    return false;
+  }
+  img.num_groups = num_groups;
+  img.memo_depth = memo_depth;
+  for (int d = 0; d < kMaxMemoDepth; d++) img.memo[d].beg.push_back(0);
+
+  for (int32_t g = 0; g < num_groups; g++) {
+    const HostGroup &hg = groups[g];
+    const int n = hg.num_states;
+    if (n < 0 || n > kIdMask + 1) {
+      std::ostringstream o;
+      o << "group " << g << " has " << n << " states; state ids are limited to 22 bits";
+      err = o.str();
+      return false;
+    }
+    if (n > 0 && (!hg.row || !hg.sets || !hg.attrs)) {
+      err = "group arrays missing";
+      return false;
+    }
+    const int E = n > 0 ? hg.row[n] : 0;
+    for (int v = 0; v < n; v++)
+      if (hg.row[v] > hg.row[v + 1] || hg.row[v] < 0) {
+        err = "CSR row offsets are not monotone";
+        return false;
+      }
+    for (int e = 0; e < E; e++)
+      if (hg.col[e] < 0 || hg.col[e] >= n) {
+        err = "CSR column index out of range";
+        return false;
+      }
+    std::vector<char> is_aas(n, 0);
+    for (int i = 0; i < hg.num_aas; i++) {
+      if (hg.aas[i] < 0 || hg.aas[i] >= n) { err = "all-input id out of range"; return false; }
+      is_aas[hg.aas[i]] = 1;
+    }
+    for (int i = 0; i < hg.num_sod; i++)
+      if (hg.sod[i] < 0 || hg.sod[i] >= n) { err = "start-of-data id out of range"; return false; }
+
+    // successor lists without all-input successors (distinct), in-degrees
+    std::vector<std::vector<int32_t>> succ(n);
+    std::vector<int32_t> indeg(n, 0);
+    uint32_t wide = 0, selfloop = 0, report = 0;
+    for (int v = 0; v < n; v++) {
+      for (int e = hg.row[v]; e < hg.row[v + 1]; e++) {
+        int u = hg.col[e];
+        if (u == v) selfloop++;
+        if (is_aas[u]) continue;
+        if (std::find(succ[v].begin(), succ[v].end(), u) == succ[v].end()) succ[v].push_back(u);
+      }
+      std::sort(succ[v].begin(), succ[v].end());
+      for (int u : succ[v]) indeg[u]++;
+      int cnt = 0;
+      for (int w = 0; w < 8; w++) cnt += __builtin_popcount(hg.sets[(size_t)v * 8 + w]);
+      if (cnt >= 128) wide++;
+      if (hg.attrs[v] & 2) report++;
+    }
+    std::vector<int32_t> tag(n);
+    for (int v = 0; v < n; v++) {
+      int32_t t = v;
+      if (hg.attrs[v] & 2) t |= kFlagReport;
+      if (succ[v].empty()) t |= kFlagDeadEnd;
+      if (indeg[v] >= 2) t |= kFlagDedup;
+      tag[v] = t;
+    }
+
+    GroupMeta gm{};
+    gm.rec_base = (uint32_t)(img.succ4.size() / 4);
+    gm.cls_base = (uint32_t)(img.cls.size() / 8);
+    gm.num_states = n;
+    gm.num_aas = hg.num_aas;
+    // records: n primary records, continuation records appended
+    size_t prim = img.succ4.size();
+    img.succ4.resize(prim + (size_t)n * 4, -1);
+    for (int v = 0; v < n; v++) {
+      const auto &s = succ[v];
+      size_t rec = prim + (size_t)v * 4;
+      size_t k = 0;
+      for (;;) {
+        size_t left = s.size() - k;
+        if (left <= 4) {
+          for (size_t i = 0; i < left; i++) img.succ4[rec + i] = tag[s[k + i]];
+          break;
+        }
+        for (size_t i = 0; i < 3; i++) img.succ4[rec + i] = tag[s[k + i]];
+        k += 3;
+        size_t next = img.succ4.size();
+        img.succ4[rec + 3] = -(int32_t)(next / 4) - 2;
+        img.succ4.resize(next + 4, -1);
+        rec = next;
+      }
+    }
+    img.cls.insert(img.cls.end(), hg.sets, hg.sets + (size_t)n * 8);
+    gm.sod_begin = (uint32_t)img.sod.size();
+    for (int i = 0; i < hg.num_sod; i++) img.sod.push_back(tag[hg.sod[i]]);
+    gm.sod_end = (uint32_t)img.sod.size();
+
+    // memo depth 1: all-input states by byte (all 256 keys present)
+    std::vector<std::vector<int32_t>> l1(256);
+    for (int i = 0; i < hg.num_aas; i++)
+      for (int b = 0; b < 256; b++)
+        if (has_byte(hg.sets, hg.aas[i], b)) l1[b].push_back(hg.aas[i]);
+    for (int b = 0; b < 256; b++) {
+      std::sort(l1[b].begin(), l1[b].end());
+      l1[b].erase(std::unique(l1[b].begin(), l1[b].end()), l1[b].end());
+      push_list(img.memo[0], l1[b], tag);
+    }
+    // deeper levels
+    std::vector<char> mark(n, 0);
+    std::vector<std::vector<int32_t>> bucket(256);
+    std::vector<std::vector<int32_t>> prev_lists;  // lists of previous depth
+    std::vector<uint32_t> prev_keys;
+    for (int b = 0; b < 256; b++) { prev_lists.push_back(l1[b]); prev_keys.push_back(b); }
+    for (int d = 2; d <= memo_depth; d++) {
+      MemoLevel &m = img.memo[d - 1];
+      gm.memo_key_base[d - 2] = (uint32_t)m.key.size();
+      std::vector<std::vector<int32_t>> cur_lists;
+      std::vector<uint32_t> cur_keys;
+      size_t dir0 = m.dir.size();
+      m.dir.resize(dir0 + 257, 0);
+      int last_first = -1;
+      for (size_t li = 0; li < prev_lists.size(); li++) {
+        std::vector<int32_t> S;
+        for (int32_t v : prev_lists[li])
+          for (int32_t u : succ[v])
+            if (!mark[u]) { mark[u] = 1; S.push_back(u); }
+        for (int32_t u : S) mark[u] = 0;
+        if (S.empty()) continue;
+        for (auto &bk : bucket) bk.clear();
+        for (int32_t u : S)
+          for (int b = 0; b < 256; b++)
+            if (has_byte(hg.sets, u, b)) bucket[b].push_back(u);
+        for (int b = 0; b < 256; b++) {
+          if (bucket[b].empty()) continue;
+          uint32_t key = (prev_keys[li] << 8) | (uint32_t)b;
+          int first = (int)(key >> (8 * (d - 1)));
+          while (last_first < first) m.dir[dir0 + (++last_first)] = (uint32_t)(m.key.size() - gm.memo_key_base[d - 2]);
+          m.key.push_back(key);
+          cur_lists.push_back(bucket[b]);
+          cur_keys.push_back(key);
+          push_list(m, bucket[b], tag);
+        }
+      }
+      uint32_t nk = (uint32_t)(m.key.size() - gm.memo_key_base[d - 2]);
+      while (last_first < 256) m.dir[dir0 + (++last_first)] = nk;
+      prev_lists.swap(cur_lists);
+      prev_keys.swap(cur_keys);
+    }
+    img.meta.push_back(gm);
+    img.stat_edges.push_back((uint32_t)E);
+    img.stat_aas.push_back(hg.num_aas);
+    img.stat_sod.push_back(hg.num_sod);
+    img.stat_report.push_back(report);
+    img.stat_wide.push_back(wide);
+    img.stat_selfloop.push_back(selfloop);
+    img.total_states += n;
+  }
+  if (img.succ4.size() / 4 >= (size_t)INT32_MAX / 2) {
+    err = "automata image too large";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace ngapamd

@@ -1,0 +1,129 @@
+"""Python mirror of the engine's C ABI (include/ngap_amd.h).
+
+Mirrors the reference launcher's flow (code/src/ngap/ngap.cu:1477-2077):
+groups are loaded once (device image + memo tables), the concatenated input
+is uploaded once, and each :meth:`Engine.run` is one timed launch whose
+reports come back sorted by (offset, id) and de-duplicated, in the reference
+encoding ``((state | group << 22) << 32) | offset``.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import ALGORITHMS, SCHED_BATCHED, SCHED_PLAIN, NgapError, check
+
+
+class Engine:
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        check(_lib.lib().ngap_create(int(device), C.byref(self._h)))
+        self.num_groups = 0
+        self.max_depth = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().ngap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def load_groups(self, groups, max_precompute_depth=2):
+        descs = (_lib.GroupDesc * len(groups))(*[g.desc() for g in groups])
+        check(_lib.lib().ngap_load_groups(self._h, descs, len(groups),
+                                          int(max_precompute_depth)))
+        self.num_groups = len(groups)
+        self.max_depth = max(1, int(max_precompute_depth))
+
+    def set_input(self, data, chunk_len, num_seg=None):
+        buf = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8)
+                                   if isinstance(data, (bytes, bytearray)) else data,
+                                   dtype=np.uint8)
+        if num_seg is None:
+            num_seg = buf.size // chunk_len
+        need = int(chunk_len) * int(num_seg)
+        if need > buf.size:
+            raise ValueError("input shorter than chunk_len * num_seg")
+        self._input = buf[:need]
+        check(_lib.lib().ngap_set_input(self._h, self._input.ctypes.data, int(chunk_len),
+                                        int(num_seg)))
+        self.chunk_len, self.num_seg = int(chunk_len), int(num_seg)
+
+    @staticmethod
+    def options(algorithm="nonblockingallgroups", schedule="plain", precompute_depth=0,
+                unique=False, is_dup=False, active_threshold=20, report_off=False,
+                result_capacity=54619400, window=0, spill_capacity=0, persistent_blocks=0,
+                use_graph=False, data_buffer_fetch_size=128, add_aas_start=0,
+                add_aas_interval=1, task_order=None):
+        o = _lib.RunOpts()
+        _lib.lib().ngap_default_opts(C.byref(o))
+        if algorithm not in ALGORITHMS:
+            raise ValueError(f"not supported algoritm {algorithm}")
+        o.algorithm = ALGORITHMS[algorithm]
+        o.schedule = SCHED_BATCHED if schedule == "batched" else SCHED_PLAIN
+        o.precompute_depth = int(precompute_depth)
+        o.unique = int(bool(unique))
+        o.is_dup = int(bool(is_dup))
+        o.active_threshold = int(active_threshold)
+        o.report_off = int(bool(report_off))
+        o.result_capacity = int(result_capacity)
+        o.window = int(window)
+        o.spill_capacity = int(spill_capacity)
+        o.persistent_blocks = int(persistent_blocks)
+        o.use_graph = int(bool(use_graph))
+        o.data_buffer_fetch_size = int(data_buffer_fetch_size)
+        o.add_aas_start = int(add_aas_start)
+        o.add_aas_interval = int(add_aas_interval)
+        if task_order is not None:
+            arr = np.ascontiguousarray(task_order, np.uint32)
+            o.task_order = arr.ctypes.data
+            o.task_order_len = arr.size
+            o._keep = arr  # keep the array alive with the options
+        return o
+
+    def run(self, opts=None, **kw):
+        """One launch.  Returns a dict of stats; raises NgapError on failure
+        (overflow included: ``err.code == NGAP_ERR_OVERFLOW``)."""
+        if opts is None:
+            opts = self.options(**kw)
+        st = _lib.RunStats()
+        code = _lib.lib().ngap_run(self._h, C.byref(opts), C.byref(st))
+        stats = dict(kernel_ms=st.kernel_ms, reports_raw=st.reports_raw,
+                     overflow_units=st.overflow_units, items=st.items,
+                     num_units=st.num_units, num_tasks=st.num_tasks)
+        if code != _lib.NGAP_OK:
+            err = NgapError(code, _lib.lib().ngap_last_error().decode(errors="replace"))
+            err.stats = stats
+            raise err
+        return stats
+
+    def reports(self):
+        """Sorted unique reports of the last run (uint64, reference encoding)."""
+        n = C.c_uint64()
+        check(_lib.lib().ngap_get_reports(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint64)
+        if n.value:
+            check(_lib.lib().ngap_get_reports(self._h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+    def task_items(self):
+        n = C.c_uint64()
+        check(_lib.lib().ngap_get_task_items(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint64)
+        if n.value:
+            check(_lib.lib().ngap_get_task_items(self._h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+
+def decode_reports(reports):
+    """(offset, group, state) arrays from reference-encoded reports."""
+    r = np.asarray(reports, np.uint64)
+    hi = (r >> np.uint64(32)).astype(np.uint32)
+    return (r & np.uint64(0xFFFFFFFF)).astype(np.uint32), hi >> 22, hi & ((1 << 22) - 1)

@@ -1,0 +1,148 @@
+// ORACLE — test infrastructure only.  Nothing in the product links, loads or
+// calls this file; only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg use it, and only as the checker.
+//
+// CPU restatement of ngAP's validator, ngap::automataGroupsReference
+// (code/src/ngap/ngap.cu:2171-2268), with set semantics (--unique=true,
+// ngap.cu:2246-2251), plus the ordering / de-duplication of
+// ngap::automataValidation (ngap.cu:2270-2344, compareResult ngap.cu:19-30).
+//
+// For group j and chunk c covering positions [cL, (c+1)L):
+//   enabled(cL)   = AAS u (SOD if c == 0 or is_dup)          ngap.cu:2204-2210
+//   matched(p)    = {v in enabled(p) : input[p] in class(v)}  ngap.cu:2226-2228
+//   report (v | j<<22, p) for reporting v in matched(p)       ngap.cu:2229-2232
+//   enabled(p+1)  = succ(matched(p)) u AAS                    ngap.cu:2233-2256
+// Output: ((state | group<<22) << 32) | offset, sorted by (offset, id), unique.
+//
+// Deliberate difference: the reference indexes its two frontiers by
+// iter % 2 with iter starting at c*L, so for odd c*L it starts on the empty
+// frontier and reports nothing for that chunk (ngap.cu:2200-2212).  The GPU
+// kernels of the reference do process such chunks; this oracle follows the
+// kernels (and the stated semantics above) and treats every chunk alike.
+//
+// It also counts the algorithmic work the roofline uses (per unit, distinct
+// items): T_ne tested non-AAS enabled states, M_aas matched AAS, M matched
+// states that advance (position < chunk end - 1), S their successor ids, R
+// reports.
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+extern "C" {
+
+struct oracle_group {
+  int32_t num_states;
+  const int32_t *row;     // [n+1]
+  const int32_t *col;     // [E]
+  const uint32_t *sets;   // [n*8]
+  const uint8_t *attrs;   // bit0 all-input, bit1 report
+  int32_t num_aas;
+  const int32_t *aas;
+  int32_t num_sod;
+  const int32_t *sod;
+};
+
+struct oracle_stats {
+  uint64_t tested_non_aas;   // T_ne
+  uint64_t matched_aas;      // M_aas
+  uint64_t matched_advance;  // M
+  uint64_t successors;       // S
+  uint64_t reports;          // R (distinct)
+  uint64_t enabled_items;    // all distinct enabled (state, pos)
+  uint64_t matched_items;    // all distinct matched (state, pos)
+};
+
+// Runs the oracle.  `unit_mask` (nullable, length num_groups*num_seg, index
+// c*num_groups + j) selects units; `threads` <= 0 means all cores.  Reports
+// are written to out (capacity cap) in reference order; *n_out gets the full
+// count even when it exceeds cap.  Returns 0, or 1 when out was too small.
+int oracle_run(int32_t num_groups, const oracle_group *groups,
+               const uint8_t *input, uint32_t chunk_len, uint32_t num_seg,
+               int32_t is_dup, const uint8_t *unit_mask, int32_t threads,
+               uint64_t *out, uint64_t cap, uint64_t *n_out,
+               oracle_stats *stats_out) {
+  const int64_t units = (int64_t)num_groups * num_seg;
+  std::vector<std::vector<uint64_t>> per_unit(units);
+  std::vector<oracle_stats> per_stats(units);
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t u = 0; u < units; u++) {
+    if (unit_mask && !unit_mask[u]) continue;
+    const int c = (int)(u / num_groups), j = (int)(u % num_groups);
+    const oracle_group &g = groups[j];
+    oracle_stats st;
+    memset(&st, 0, sizeof st);
+    std::vector<uint64_t> &rep = per_unit[u];
+    std::vector<char> is_aas(g.num_states, 0);
+    for (int i = 0; i < g.num_aas; i++) is_aas[g.aas[i]] = 1;
+    std::vector<int32_t> cur, nxt;
+    if (c == 0 || is_dup) cur.assign(g.sod, g.sod + g.num_sod);
+    cur.insert(cur.end(), g.aas, g.aas + g.num_aas);
+    std::sort(cur.begin(), cur.end());
+    cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+    const uint64_t begin = (uint64_t)c * chunk_len, end = begin + chunk_len;
+    for (uint64_t p = begin; p < end && !cur.empty(); p++) {
+      const uint8_t b = input[p];
+      nxt.clear();
+      st.enabled_items += cur.size();
+      for (int32_t v : cur) {
+        if (!is_aas[v]) st.tested_non_aas++;
+        if (!((g.sets[(size_t)v * 8 + b / 32] >> (b % 32)) & 1u)) continue;
+        st.matched_items++;
+        if (is_aas[v]) st.matched_aas++;
+        if (g.attrs[v] & 2) {
+          uint32_t id = (uint32_t)v | ((uint32_t)j << 22);
+          rep.push_back(((uint64_t)id << 32) | (uint32_t)p);
+        }
+        if (p + 1 < end) {
+          st.matched_advance++;
+          st.successors += (uint64_t)(g.row[v + 1] - g.row[v]);
+        }
+        for (int e = g.row[v]; e < g.row[v + 1]; e++) nxt.push_back(g.col[e]);
+      }
+      nxt.insert(nxt.end(), g.aas, g.aas + g.num_aas);
+      std::sort(nxt.begin(), nxt.end());
+      nxt.erase(std::unique(nxt.begin(), nxt.end()), nxt.end());
+      cur.swap(nxt);
+    }
+    st.reports = rep.size();
+    per_stats[u] = st;
+  }
+  std::vector<uint64_t> all;
+  oracle_stats tot;
+  memset(&tot, 0, sizeof tot);
+  size_t total = 0;
+  for (int64_t u = 0; u < units; u++) total += per_unit[u].size();
+  all.reserve(total);
+  for (int64_t u = 0; u < units; u++) {
+    all.insert(all.end(), per_unit[u].begin(), per_unit[u].end());
+    std::vector<uint64_t>().swap(per_unit[u]);
+    tot.tested_non_aas += per_stats[u].tested_non_aas;
+    tot.matched_aas += per_stats[u].matched_aas;
+    tot.matched_advance += per_stats[u].matched_advance;
+    tot.successors += per_stats[u].successors;
+    tot.reports += per_stats[u].reports;
+    tot.enabled_items += per_stats[u].enabled_items;
+    tot.matched_items += per_stats[u].matched_items;
+  }
+  // compareResult order: by offset (low word), then by id (high word)
+  std::sort(all.begin(), all.end(), [](uint64_t a, uint64_t b) {
+    uint32_t ia = (uint32_t)a, ib = (uint32_t)b;
+    if (ia != ib) return ia < ib;
+    return (a >> 32) < (b >> 32);
+  });
+  all.erase(std::unique(all.begin(), all.end()), all.end());
+  *n_out = all.size();
+  if (stats_out) *stats_out = tot;
+  if (all.size() > cap) {
+    if (out) std::copy(all.begin(), all.begin() + cap, out);
+    return 1;
+  }
+  if (out) std::copy(all.begin(), all.end(), out);
+  return 0;
+}
+
+}  // extern "C"

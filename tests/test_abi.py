@@ -1,0 +1,51 @@
+"""The C-ABI library loads and exports every symbol include/ngap_amd.h
+declares; the front-end entry points work without a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from ngap_amd import _lib
+from tests.conftest import ROOT, have_gpu
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "ngap_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ngap_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(_lib.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # the Python mirror binds each of them
+    assert set(syms) <= set(_lib.SIGNATURES), set(syms) - set(_lib.SIGNATURES)
+
+
+def test_default_options_follow_reference_defaults():
+    o = _lib.RunOpts()
+    _lib.lib().ngap_default_opts(C.byref(o))
+    # ngap_option.h:52-68, common_func.h:50-58
+    assert o.data_buffer_fetch_size == 128 and o.add_aas_start == 0
+    assert o.add_aas_interval == 1 and o.active_threshold == 20
+    assert o.unique == 0 and o.precompute_depth == 0
+    assert o.result_capacity == 54619400
+
+
+@pytest.mark.skipif(have_gpu(), reason="checks the no-device error path")
+def test_create_without_device_fails_cleanly():
+    h = C.c_void_p()
+    code = _lib.lib().ngap_create(0, C.byref(h))
+    assert code == _lib.NGAP_ERR_DEVICE
+    assert b"device" in _lib.lib().ngap_last_error()
+
+
+def test_null_arguments_are_errors():
+    L = _lib.lib()
+    assert L.ngap_run(None, None, None) == _lib.NGAP_ERR_INVALID
+    assert L.ngap_frontend_load(None, 1, -1, 1, 0, None) == _lib.NGAP_ERR_INVALID
+    assert L.ngap_frontend_free(None) == _lib.NGAP_OK

@@ -1,0 +1,173 @@
+"""HIP engine vs the oracle: bit-exact sorted unique report sets.
+
+Every case prints its name before touching the GPU.  Sizes keep the oracle
+to seconds; the full-size workloads are checked through properties in
+test_gpu_fullsize.py.
+"""
+import os
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from ngap_amd import Engine, Frontend, NgapError, synth
+from ngap_amd.groups import make_group
+from tests.conftest import GOLDEN
+from tests import nfa_helpers as H
+
+pytestmark = pytest.mark.gpu
+
+ALGOS = ["NAPgroups", "nonblockinggroups", "nonblockingr1groups", "nonblockingr2groups",
+         "nonblockingpcgroups", "nonblockingallgroups"]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def check_all(engine, groups, data, chunk, num_seg=None, configs=None, is_dup=False, depth=3):
+    want, _ = oracle.run(groups, data, chunk, num_seg, is_dup=is_dup)
+    engine.load_groups(groups, max_precompute_depth=depth)
+    engine.set_input(data, chunk, num_seg)
+    configs = configs or [dict(algorithm=a) for a in ALGOS]
+    for cfg in configs:
+        cfg = dict(cfg)
+        cfg.setdefault("is_dup", is_dup)
+        say(cfg)
+        engine.run(**cfg)
+        got = engine.reports()
+        assert got.shape == want.shape and np.array_equal(got, want), \
+            (cfg, got.size, want.size)
+    return want
+
+
+def test_apple(engine):
+    g = Frontend.load(os.path.join(GOLDEN, "apple.anml"), 1).groups()
+    data = open(os.path.join(GOLDEN, "apple_input.txt"), "rb").read()
+    cfgs = []
+    for a in ALGOS:
+        for sched in ("plain", "batched"):
+            for d in (0, 1, 2, 3):
+                cfgs.append(dict(algorithm=a, schedule=sched, precompute_depth=d, unique=d % 2))
+    want = check_all(engine, g, data, 81, 1, cfgs)
+    assert [hex(x) for x in want] == ["0x400000005", "0x40000002f", "0x400000040", "0x40000004b"]
+
+
+def test_literal64_small(engine):
+    groups, data, _ = synth.literal64(input_len=1 << 16)
+    cfgs = [dict(algorithm=a, schedule=s, precompute_depth=d)
+            for a in ("nonblockinggroups", "nonblockingpcgroups", "nonblockingallgroups")
+            for s in ("plain", "batched") for d in (1, 2, 3)]
+    check_all(engine, groups, data, 1 << 16, 1, cfgs)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_regex_patterns_multi_chunk(engine, seed):
+    rng = np.random.default_rng(seed)
+    g = synth.regex_group(rng, 20000, n_patterns=800)
+    data = synth.zipf_bytes(rng, 1 << 18, 1.1)
+    cfgs = [dict(algorithm=a, schedule=s, precompute_depth=d, unique=u, window=w)
+            for a, s, d, u, w in [("nonblockingallgroups", "plain", 2, 1, 0),
+                                  ("nonblockingallgroups", "batched", 2, 1, 4096),
+                                  ("nonblockingallgroups", "batched", 3, 0, 1000),
+                                  ("nonblockingpcgroups", "plain", 3, 1, 0),
+                                  ("nonblockingr1groups", "batched", 1, 0, 8192),
+                                  ("NAPgroups", "plain", 0, 0, 0)]]
+    check_all(engine, [g], data, 1 << 14, None, cfgs)
+
+
+def test_many_groups_with_dot_star(engine):
+    rng = np.random.default_rng(11)
+    groups = [synth.regex_group(rng, int(s), dotstar=(i % 5 == 0))
+              for i, s in enumerate(np.exp(rng.uniform(np.log(50), np.log(3000), 40)))]
+    data = synth.zipf_bytes(rng, 1 << 17, 1.1)
+    cfgs = [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d, unique=1)
+            for s in ("plain", "batched") for d in (1, 2, 3)]
+    check_all(engine, groups, data, 1 << 13, None, cfgs)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_class_repeat_chains(engine, seed):
+    r = random.Random(200 + seed)
+    alpha = list(b"abcd")
+    specs = [H.random_spec(r, r.randint(2, 6), alpha) for _ in range(30)]
+    g, _ = H.chains_group(specs)
+    data = bytes(r.choice(alpha) for _ in range(5000))
+    cfgs = [dict(algorithm=a, schedule=s, precompute_depth=d, unique=u)
+            for a in ("nonblockinggroups", "nonblockingallgroups")
+            for s in ("plain", "batched") for d in (1, 3) for u in (0, 1)]
+    check_all(engine, [g], data, 250, None, cfgs)
+
+
+@pytest.mark.parametrize("is_dup", [False, True])
+def test_start_of_data_chunks(engine, is_dup):
+    r = random.Random(31)
+    alpha = list(b"ab")
+    specs = [H.random_spec(r, 3, alpha) for _ in range(6)]
+    g, _ = H.chains_group(specs, start="start-of-data")
+    data = bytes(r.choice(alpha) for _ in range(640))
+    cfgs = [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d)
+            for s in ("plain", "batched") for d in (1, 2)]
+    check_all(engine, [g], data, 64, None, cfgs, is_dup=is_dup)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_anml_with_degree_splits(engine, tmp_path, seed):
+    path = tmp_path / f"rand{seed}.anml"
+    from ngap_amd.groups import write_anml
+    write_anml(path, H.random_anml_states(seed))
+    groups = Frontend.load(str(path), 3).groups()
+    r = random.Random(seed)
+    data = bytes(r.choice(b"abcxyzAB019\n\t -[]\\") for _ in range(3000))
+    cfgs = [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d, unique=1)
+            for s in ("plain", "batched") for d in (1, 2, 3)]
+    check_all(engine, groups, data, 300, None, cfgs)
+
+
+def test_wide_fanout_direct_api(engine):
+    # states with more than four successors (continuation records)
+    n = 40
+    edges = [(0, j) for j in range(1, 12)] + [(j, 12 + (j % 5)) for j in range(1, 12)]
+    edges += [(12 + k, 20 + k) for k in range(5)] + [(20, 20)]
+    classes = [[ord("a")]] + [[ord("b"), ord("c")]] * 11 + [[ord("c"), ord("d")]] * 8 + \
+        [list(range(256))] * (n - 20)
+    g = make_group(n, edges, classes, all_input=[0], reporting=list(range(12, 25)))
+    data = b"abcdcdab" * 300
+    cfgs = [dict(algorithm=a, schedule=s, precompute_depth=d, unique=u)
+            for a in ("nonblockinggroups", "nonblockingallgroups")
+            for s in ("plain", "batched") for d in (1, 2, 3) for u in (0, 1)]
+    check_all(engine, [g], data, 600, None, cfgs)
+
+
+def test_chunk_of_one_byte_and_no_reports(engine):
+    g = make_group(2, [(0, 1)], [[ord("a")], [ord("b")]], all_input=[0], reporting=[1])
+    check_all(engine, [g], b"ab" * 64, 1, None,
+              [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d)
+               for s in ("plain", "batched") for d in (1, 3)])
+    check_all(engine, [g], b"zz" * 64, 16, None,
+              [dict(algorithm="nonblockingallgroups", schedule="plain")])
+
+
+def test_overflow_is_reported_not_wrong(engine):
+    rng = np.random.default_rng(5)
+    g = synth.regex_group(rng, 5000, dotstar=False)
+    data = synth.zipf_bytes(rng, 1 << 16, 0.6)
+    engine.load_groups([g], 2)
+    engine.set_input(data, 1 << 16, 1)
+    say("tiny result capacity")
+    with pytest.raises(NgapError) as ei:
+        engine.run(algorithm="nonblockingallgroups", precompute_depth=2, result_capacity=1)
+    assert ei.value.code == 4
+
+
+def test_reports_bit_exact_after_reload(engine):
+    # reloading groups / input between runs keeps results exact
+    rng = np.random.default_rng(9)
+    for k in range(3):
+        g = synth.regex_group(rng, 3000 * (k + 1))
+        data = synth.zipf_bytes(rng, 1 << 15, 1.1)
+        check_all(engine, [g], data, 1 << 13, None,
+                  [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2)])
