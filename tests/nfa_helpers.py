@@ -98,9 +98,11 @@ def reports_to_pattern_offsets(reports, owner, group_index=0):
     return {(owner[int(s)], int(o)) for s, o in zip(st, off)}
 
 
-def random_anml_states(seed, n=60, max_deg=9, n_cc_hint=4, self_loop_p=0.15):
-    """Random ANML states (dicts for write_anml) with out-degrees above 4,
-    self-loops, several components and assorted symbol-set spellings."""
+def random_anml_states(seed, n=60, n_cc_hint=4, self_loop_p=0.15, wide_p=0.15):
+    """Random ANML states (dicts for write_anml): mostly forward edges, some
+    states with 5..8 successors (so the out-degree limit splits them), a few
+    back edges and self-loops, several components, assorted symbol-set
+    spellings."""
     r = random.Random(seed)
     spellings = ["*", ".", "[a-c]", "[^a]", "\\x41", "[\\x00-\\x1f]", "\\d", "[\\w]", "\\s",
                  "[a\\-z]", "[-a]", "[xyz]", "a", "[\\x30-\\x39A]", "[^\\x0a]", "[\\n\\t]",
@@ -108,9 +110,12 @@ def random_anml_states(seed, n=60, max_deg=9, n_cc_hint=4, self_loop_p=0.15):
     comp = [r.randrange(n_cc_hint) for _ in range(n)]
     states = []
     for i in range(n):
-        same = [j for j in range(n) if comp[j] == comp[i] and j != i]
-        deg = r.randint(0, max_deg)
-        nxt = r.sample(same, min(deg, len(same)))
+        later = [j for j in range(i + 1, n) if comp[j] == comp[i]]
+        deg = r.randint(5, 8) if r.random() < wide_p else r.randint(0, 2)
+        nxt = r.sample(later, min(deg, len(later)))
+        earlier = [j for j in range(i) if comp[j] == comp[i]]
+        if earlier and r.random() < 0.1:
+            nxt.append(r.choice(earlier))
         if r.random() < self_loop_p:
             nxt.insert(r.randrange(len(nxt) + 1), i)
         st = r.choice(["", "", "", "all-input", "start-of-data"])
