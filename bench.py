@@ -1,0 +1,208 @@
+"""Benchmark: input MB/s of the non-blocking automata engine on one MI355X.
+
+Default workload (regex20k): 20,000 random regex patterns (~474k states, one
+group, all components rooted at all-input states), 16 MiB of Zipf(1.1) input
+in 256 KiB chunks.  A step is one launch over the whole input (plain schedule:
+one workgroup per (group, chunk) unit; batched: persistent waves over 4 KiB
+work items), inputs resident in HBM, reports kept on the device.
+
+Printed JSON (one line): value = input MB/s of the batched engine (wall clock
+over the timed steps, each step = counter reset + matching launch + overflow
+check), plus the plain engine's MB/s, a bit-exactness check (batched == plain
+in full, both == oracle on sampled chunks), the roofline of the dominant
+kernel and the CPU baseline (the oracle timed on host cores).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "input MB/s, bit-exact reports: batched+learned vs plain ngAP; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def hip_sync():
+    hip = C.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def workload(name):
+    from ngap_amd import synth
+    if name == "regex20k":
+        groups, data, chunk = synth.regex20k()
+    elif name == "literal64":
+        groups, data, chunk = synth.literal64()
+    elif name == "batch1024":
+        groups, data, chunk = synth.batch1024()
+    elif name == "learned":
+        groups, data, chunk = synth.learned()
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return groups, data, chunk
+
+
+def sample_stats(groups, data, chunk, n_units=4, sample_len=16384, threads=0):
+    """Oracle work counts on the first sample_len bytes of n_units chunks,
+    scaled to bytes per input byte (chunk-start effects are negligible)."""
+    import oracle
+    S = len(data) // chunk
+    picks = np.linspace(0, S - 1, min(n_units, S)).astype(int)
+    sample = np.concatenate([data[c * chunk: c * chunk + min(sample_len, chunk)] for c in picks])
+    L = min(sample_len, chunk)
+    t = time.time()
+    _, st = oracle.run(groups, sample, L, len(picks), threads=threads)
+    dt = time.time() - t
+    per_byte = oracle.algorithmic_bytes(st, sample.size) / sample.size
+    return per_byte, st, sample.size, dt, len(picks)
+
+
+def cpu_baseline(groups, data, chunk, budget_s=15.0):
+    """The oracle (a port of the reference's CPU validator) timed on host
+    cores over a bounded sample: one unit per core, each a slice of a chunk."""
+    import oracle
+    cores = min(16, os.cpu_count() or 1)
+    # calibrate on 2 KiB of one unit
+    L0 = min(2048, chunk)
+    t = time.time()
+    oracle.run(groups, data[:L0], L0, 1, threads=1)
+    rate = L0 / max(1e-6, time.time() - t)  # bytes/s per core
+    L = int(min(chunk, max(256, rate * budget_s)))
+    S = len(data) // chunk
+    units = min(cores, S)
+    sample = np.concatenate([data[c * chunk: c * chunk + L] for c in range(units)])
+    t = time.time()
+    oracle.run(groups, sample, L, units, threads=units)
+    dt = time.time() - t
+    return dict(value=round(sample.size / dt / 1e6, 6), unit="MB/s", cores=units, kind="port",
+                sample=f"{units} units x {L} bytes (first bytes of chunks 0..{units - 1}), "
+                       f"{dt:.1f} s, oracle with unique=true semantics")
+
+
+def timed(eng, opts, steps, warmup):
+    for _ in range(warmup):
+        eng.run(opts)
+    hip_sync()
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = eng.run(opts)
+        ks.append(st["kernel_ms"])
+    hip_sync()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    return wall, float(np.mean(ks)), st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="regex20k")
+    ap.add_argument("--depth", type=int, default=2)
+    ap.add_argument("--window", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--only", choices=["plain", "batched"], default=None)
+    ap.add_argument("--oracle-check", action="store_true",
+                    help="also compare reports with the oracle on three full chunks (slow)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the (group, chunk) units would shard across GPUs with no exchange)"}))
+        sys.exit(2)
+
+    from ngap_amd import Engine
+    t = time.time()
+    groups, data, chunk = workload(a.workload)
+    total = (len(data) // chunk) * chunk
+    log(f"workload {a.workload}: {len(groups)} groups, {sum(g.num_states for g in groups)} states, "
+        f"{total} bytes, chunk {chunk} ({time.time() - t:.1f} s)")
+    eng = Engine(0)
+    t = time.time()
+    eng.load_groups(groups, max_precompute_depth=max(a.depth, 1))
+    eng.set_input(data, chunk)
+    log(f"loaded ({time.time() - t:.1f} s)")
+
+    common = dict(algorithm="nonblockingallgroups", precompute_depth=a.depth, unique=True)
+    res = {}
+    for sched in (["plain", "batched"] if a.only is None else [a.only]):
+        opts = eng.options(schedule=sched, window=a.window, **common)
+        wall, kern, st = timed(eng, opts, a.steps, a.warmup)
+        res[sched] = dict(wall_ms=wall, kernel_ms=kern, stats=st, reports=eng.reports())
+        log(f"{sched}: {wall:.3f} ms/step wall, {kern:.3f} ms kernel, "
+            f"{total / wall / 1e3:.1f} MB/s, items {st['items']}, reports {res[sched]['reports'].size}")
+
+    main_s = "batched" if "batched" in res else "plain"
+    bit_exact = None
+    if not a.no_check and len(res) == 2:
+        bit_exact = bool(np.array_equal(res["plain"]["reports"], res["batched"]["reports"]))
+    oracle_check = None
+    per_byte = None
+    if not a.no_check:
+        per_byte, st, nbytes, dt, nunits = sample_stats(groups, data, chunk)
+        log(f"oracle sample: {nbytes} bytes over {nunits} chunks in {dt:.1f} s, "
+            f"{per_byte:.1f} algorithmic bytes per input byte")
+        # reports on sampled chunks: engine == oracle restricted to those chunks
+        import oracle
+        S = len(data) // chunk
+        mask = np.zeros(len(groups) * S, np.uint8)
+        picks = [0, S // 2, S - 1]
+        for c in picks:
+            mask[c * len(groups):(c + 1) * len(groups)] = 1
+        if a.oracle_check or a.workload == "literal64":
+            want, _ = oracle.run(groups, data, chunk, S, unit_mask=mask)
+            got = res[main_s]["reports"]
+            off = (got & np.uint64(0xFFFFFFFF)).astype(np.int64)
+            keep = np.isin(off // chunk, picks)
+            oracle_check = bool(np.array_equal(got[keep], want))
+            log(f"oracle check on chunks {picks}: {oracle_check}")
+
+    wall = res[main_s]["wall_ms"]
+    kern = res[main_s]["kernel_ms"]
+    value = total / (wall * 1e-3) / 1e6
+    roofline = None
+    if per_byte is not None:
+        achieved = per_byte * total / (kern * 1e-3) / 1e9
+        roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
+                        kernel="k_batched" if main_s == "batched" else "k_plain",
+                        algorithmic_bytes_per_input_byte=round(per_byte, 2))
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "MB/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (seeded generators, ngap_amd/synth.py)",
+        "config": {"workload": a.workload, "groups": len(groups),
+                   "states": int(sum(g.num_states for g in groups)), "input_bytes": int(total),
+                   "chunk": int(chunk), "algorithm": "nonblockingallgroups",
+                   "precompute_depth": a.depth, "window": a.window},
+        "kernel_ms": round(kern, 4), "kernel_mbps": round(total / (kern * 1e-3) / 1e6, 3),
+        "plain_mbps": round(total / (res["plain"]["wall_ms"] * 1e-3) / 1e6, 3) if "plain" in res else None,
+        "batched_mbps": round(total / (res["batched"]["wall_ms"] * 1e-3) / 1e6, 3) if "batched" in res else None,
+        "speedup_batched_vs_plain": round(res["plain"]["wall_ms"] / res["batched"]["wall_ms"], 3)
+        if len(res) == 2 else None,
+        "reports": int(res[main_s]["reports"].size),
+        "bit_exact_batched_vs_plain": bit_exact, "bit_exact_vs_oracle_sampled": oracle_check,
+        "roofline": roofline,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(groups, data, chunk)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
