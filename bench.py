@@ -54,7 +54,7 @@ def workload(name):
     return groups, data, chunk
 
 
-def sample_stats(groups, data, chunk, n_units=4, sample_len=16384, threads=0):
+def sample_stats(groups, data, chunk, depth, n_units=4, sample_len=16384, threads=0):
     """Oracle work counts on the first sample_len bytes of n_units chunks,
     scaled to bytes per input byte (chunk-start effects are negligible)."""
     import oracle
@@ -63,10 +63,11 @@ def sample_stats(groups, data, chunk, n_units=4, sample_len=16384, threads=0):
     sample = np.concatenate([data[c * chunk: c * chunk + min(sample_len, chunk)] for c in picks])
     L = min(sample_len, chunk)
     t = time.time()
-    _, st = oracle.run(groups, sample, L, len(picks), threads=threads)
+    _, st = oracle.run(groups, sample, L, len(picks), threads=threads, memo_depth=depth)
     dt = time.time() - t
     per_byte = oracle.algorithmic_bytes(st, sample.size) / sample.size
-    return per_byte, st, sample.size, dt, len(picks)
+    per_byte_ref = oracle.algorithmic_bytes_reference_model(st, sample.size) / sample.size
+    return per_byte, per_byte_ref, st, sample.size, dt, len(picks)
 
 
 def cpu_baseline(groups, data, chunk, budget_s=15.0):
@@ -152,9 +153,11 @@ def main():
     oracle_check = None
     per_byte = None
     if not a.no_check:
-        per_byte, st, nbytes, dt, nunits = sample_stats(groups, data, chunk)
+        per_byte, per_byte_ref, st, nbytes, dt, nunits = sample_stats(groups, data, chunk,
+                                                                        max(1, a.depth))
         log(f"oracle sample: {nbytes} bytes over {nunits} chunks in {dt:.1f} s, "
-            f"{per_byte:.1f} algorithmic bytes per input byte")
+            f"{per_byte:.2f} algorithmic bytes per input byte at memo depth {a.depth} "
+            f"({per_byte_ref:.1f} for the reference algorithm); {st}")
         # reports on sampled chunks: engine == oracle restricted to those chunks
         import oracle
         S = len(data) // chunk
@@ -179,7 +182,8 @@ def main():
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
                         kernel="k_batched" if main_s == "batched" else "k_plain",
-                        algorithmic_bytes_per_input_byte=round(per_byte, 2))
+                        algorithmic_bytes_per_input_byte=round(per_byte, 3),
+                        reference_model_bytes_per_input_byte=round(per_byte_ref, 2))
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "MB/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall, 4),

@@ -26,7 +26,8 @@ class OracleGroup(C.Structure):
 class OracleStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "tested_non_aas", "matched_aas", "matched_advance", "successors", "reports",
-        "enabled_items", "matched_items")]
+        "enabled_items", "matched_items", "memo_list_items", "expanded_items",
+        "expanded_tests")]
 
 
 _lib = None
@@ -44,13 +45,15 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.oracle_run.restype = C.c_int
         L.oracle_run.argtypes = [C.c_int32, C.POINTER(OracleGroup), C.c_void_p, C.c_uint32,
-                                 C.c_uint32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
-                                 C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(OracleStats)]
+                                 C.c_uint32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                                 C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                 C.POINTER(OracleStats)]
         _lib = L
     return _lib
 
 
-def run(groups, data, chunk_len, num_seg=None, is_dup=False, threads=0, unit_mask=None):
+def run(groups, data, chunk_len, num_seg=None, is_dup=False, threads=0, unit_mask=None,
+        memo_depth=1):
     """Reports (uint64 array, reference encoding, sorted unique) and stats."""
     buf = np.frombuffer(bytes(data), np.uint8) if isinstance(data, (bytes, bytearray)) \
         else np.ascontiguousarray(data, np.uint8)
@@ -71,20 +74,31 @@ def run(groups, data, chunk_len, num_seg=None, is_dup=False, threads=0, unit_mas
     st = OracleStats()
     L = lib()
     L.oracle_run(len(groups), og, buf.ctypes.data, chunk_len, num_seg, int(is_dup),
-                 mask.ctypes.data if mask is not None else None, int(threads), None, 0,
-                 C.byref(n), C.byref(st))
+                 mask.ctypes.data if mask is not None else None, int(threads), int(memo_depth),
+                 None, 0, C.byref(n), C.byref(st))
     out = np.zeros(n.value, np.uint64)
     L.oracle_run(len(groups), og, buf.ctypes.data, chunk_len, num_seg, int(is_dup),
-                 mask.ctypes.data if mask is not None else None, int(threads),
+                 mask.ctypes.data if mask is not None else None, int(threads), int(memo_depth),
                  out.ctypes.data, n.value, C.byref(n), C.byref(st))
     stats = {k: getattr(st, k) for k, _ in OracleStats._fields_}
     return out, stats
 
 
-def algorithmic_bytes(stats, total_input_bytes):
-    """Bytes the path must move at minimum: L + 4*T_ne + 4*M_aas + 8*M + 4*S + 8*R."""
+def algorithmic_bytes_reference_model(stats, total_input_bytes):
+    """The reference algorithm's bytes (every all-input state tested at every
+    position): L + 4*T_ne + 4*M_aas + 8*M + 4*S + 8*R."""
     return (total_input_bytes + 4 * stats["tested_non_aas"] + 4 * stats["matched_aas"]
             + 8 * stats["matched_advance"] + 4 * stats["successors"] + 8 * stats["reports"])
+
+
+def algorithmic_bytes(stats, total_input_bytes):
+    """Bytes a prefix-memoized engine (depth k, the stats' memo_depth) must move
+    at minimum: per input byte 1 B input + 8 B memo index entry; 4 B per memo
+    list entry materialised; per expanded item one 16 B successor record and a
+    4 B class word per non-all-input successor; 8 B per report."""
+    return (9 * total_input_bytes + 4 * stats["memo_list_items"]
+            + 16 * stats["expanded_items"] + 4 * stats["expanded_tests"]
+            + 8 * stats["reports"])
 
 
 def ref_dump(anml_path, group_num, max_nfa_size=-1):

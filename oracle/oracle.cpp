@@ -23,7 +23,13 @@
 // It also counts the algorithmic work the roofline uses (per unit, distinct
 // items): T_ne tested non-AAS enabled states, M_aas matched AAS, M matched
 // states that advance (position < chunk end - 1), S their successor ids, R
-// reports.
+// reports.  For a prefix-memoized engine of depth k it further tracks, per
+// matched item, the longest chain from an all-input root (capped at k; items
+// seeded from start-of-data count as k): an item whose every chain is shorter
+// than k-1 is covered by the memo tables and never materialised; an item
+// with depth k-1 is read from a memo list; materialised items that advance
+// are expanded (one successor record, one class word per non-all-input
+// successor).
 #include <omp.h>
 
 #include <algorithm>
@@ -53,6 +59,9 @@ struct oracle_stats {
   uint64_t reports;          // R (distinct)
   uint64_t enabled_items;    // all distinct enabled (state, pos)
   uint64_t matched_items;    // all distinct matched (state, pos)
+  uint64_t memo_list_items;  // materialised from a depth-k memo list
+  uint64_t expanded_items;   // materialised items that advance
+  uint64_t expanded_tests;   // their non-all-input successors
 };
 
 // Runs the oracle.  `unit_mask` (nullable, length num_groups*num_seg, index
@@ -62,8 +71,9 @@ struct oracle_stats {
 int oracle_run(int32_t num_groups, const oracle_group *groups,
                const uint8_t *input, uint32_t chunk_len, uint32_t num_seg,
                int32_t is_dup, const uint8_t *unit_mask, int32_t threads,
-               uint64_t *out, uint64_t cap, uint64_t *n_out,
+               int32_t memo_depth, uint64_t *out, uint64_t cap, uint64_t *n_out,
                oracle_stats *stats_out) {
+  const int K = memo_depth < 1 ? 1 : memo_depth;
   const int64_t units = (int64_t)num_groups * num_seg;
   std::vector<std::vector<uint64_t>> per_unit(units);
   std::vector<oracle_stats> per_stats(units);
@@ -78,17 +88,33 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
     std::vector<uint64_t> &rep = per_unit[u];
     std::vector<char> is_aas(g.num_states, 0);
     for (int i = 0; i < g.num_aas; i++) is_aas[g.aas[i]] = 1;
-    std::vector<int32_t> cur, nxt;
-    if (c == 0 || is_dup) cur.assign(g.sod, g.sod + g.num_sod);
-    cur.insert(cur.end(), g.aas, g.aas + g.num_aas);
-    std::sort(cur.begin(), cur.end());
-    cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+    std::vector<int32_t> nonaas_deg(g.num_states, 0);
+    for (int v = 0; v < g.num_states; v++)
+      for (int e = g.row[v]; e < g.row[v + 1]; e++) nonaas_deg[v] += !is_aas[g.col[e]];
+    // frontier entries: (state, root depth capped at K); duplicates merged
+    // keeping the deepest chain
+    typedef std::pair<int32_t, int32_t> SD;
+    std::vector<SD> cur, nxt;
+    auto merge = [](std::vector<SD> &f) {
+      std::sort(f.begin(), f.end(), [](const SD &a, const SD &b) {
+        return a.first != b.first ? a.first < b.first : a.second > b.second;
+      });
+      size_t w = 0;
+      for (size_t i = 0; i < f.size(); i++)
+        if (w == 0 || f[w - 1].first != f[i].first) f[w++] = f[i];
+      f.resize(w);
+    };
+    if (c == 0 || is_dup)
+      for (int i = 0; i < g.num_sod; i++) cur.emplace_back(g.sod[i], K);
+    for (int i = 0; i < g.num_aas; i++) cur.emplace_back(g.aas[i], 0);
+    merge(cur);
     const uint64_t begin = (uint64_t)c * chunk_len, end = begin + chunk_len;
     for (uint64_t p = begin; p < end && !cur.empty(); p++) {
       const uint8_t b = input[p];
       nxt.clear();
       st.enabled_items += cur.size();
-      for (int32_t v : cur) {
+      for (const SD &vd : cur) {
+        const int32_t v = vd.first, d = vd.second;
         if (!is_aas[v]) st.tested_non_aas++;
         if (!((g.sets[(size_t)v * 8 + b / 32] >> (b % 32)) & 1u)) continue;
         st.matched_items++;
@@ -97,15 +123,21 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
           uint32_t id = (uint32_t)v | ((uint32_t)j << 22);
           rep.push_back(((uint64_t)id << 32) | (uint32_t)p);
         }
+        const bool materialised = d >= K - 1;
+        if (d == K - 1) st.memo_list_items++;
         if (p + 1 < end) {
           st.matched_advance++;
           st.successors += (uint64_t)(g.row[v + 1] - g.row[v]);
+          if (materialised && nonaas_deg[v] > 0) {
+            st.expanded_items++;
+            st.expanded_tests += nonaas_deg[v];
+          }
         }
-        for (int e = g.row[v]; e < g.row[v + 1]; e++) nxt.push_back(g.col[e]);
+        const int32_t dn = d + 1 < K ? d + 1 : K;
+        for (int e = g.row[v]; e < g.row[v + 1]; e++) nxt.emplace_back(g.col[e], dn);
       }
-      nxt.insert(nxt.end(), g.aas, g.aas + g.num_aas);
-      std::sort(nxt.begin(), nxt.end());
-      nxt.erase(std::unique(nxt.begin(), nxt.end()), nxt.end());
+      for (int i = 0; i < g.num_aas; i++) nxt.emplace_back(g.aas[i], 0);
+      merge(nxt);
       cur.swap(nxt);
     }
     st.reports = rep.size();
@@ -127,6 +159,9 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
     tot.reports += per_stats[u].reports;
     tot.enabled_items += per_stats[u].enabled_items;
     tot.matched_items += per_stats[u].matched_items;
+    tot.memo_list_items += per_stats[u].memo_list_items;
+    tot.expanded_items += per_stats[u].expanded_items;
+    tot.expanded_tests += per_stats[u].expanded_tests;
   }
   // compareResult order: by offset (low word), then by id (high word)
   std::sort(all.begin(), all.end(), [](uint64_t a, uint64_t b) {

@@ -113,3 +113,18 @@ def test_work_counts_on_a_closed_form():
     assert st["reports"] == n
     assert st["matched_advance"] == 2 * n - 1  # all but the final 'b' advance
     assert st["successors"] == n  # only the 'a' items have a successor
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_memo_work_counts_on_a_closed_form(k):
+    # all-input 'a' -> 'b' -> 'c' (report) over "abc"*n: every chain is a-b-c.
+    # depth k: states at chain depth k-1 come from memo lists; items at chain
+    # depth >= k-1 that have a successor are expanded.
+    n = 40
+    g = make_group(3, [(0, 1), (1, 2)], [[ord("a")], [ord("b")], [ord("c")]],
+                   all_input=[0], reporting=[2])
+    _, st = oracle.run([g], b"abc" * n, 3 * n, 1, memo_depth=k)
+    assert st["memo_list_items"] == n              # one chain depth per k (0, 1, 2)
+    assert st["expanded_items"] == {1: 2 * n, 2: n, 3: 0}[k]
+    assert st["expanded_tests"] == st["expanded_items"]
+    assert st["reports"] == n
