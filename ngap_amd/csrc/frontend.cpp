@@ -613,6 +613,13 @@ bool build_groups(const Nfa &nfa, FrontendOptions &opt,
   }
   std::vector<Nfa> ccs = split_by_cc(nfa);
   st.num_cc = (int)ccs.size();
+  long long sum = 0;
+  for (auto &c : ccs) {
+    sum += c.size();
+    st.max_cc = std::max(st.max_cc, c.size());
+    if (c.size() > 256) st.cc_256++;
+  }
+  st.avg_cc = ccs.empty() ? 0.0 : (double)sum / ccs.size();
   // ngap::preprocessing (code/src/ngap/ngap.cu:63-90)
   std::vector<Nfa> kept;
   for (size_t i = 0; i < ccs.size(); i++) {

@@ -66,6 +66,8 @@ struct FrontendOptions {
 struct FrontendStats {
   int total_states = 0, start_states = 0, always_active = 0, report_states = 0;
   int num_cc = 0, limit_succeed = 0, limit_fail = 0, removed_by_size = 0;
+  int cc_256 = 0, max_cc = 0;   // components > 256 states, largest component
+  double avg_cc = 0;
 };
 
 // Parses a VASim symbol-set string.  Returns false (and fills err) where the
