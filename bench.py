@@ -92,6 +92,22 @@ def cpu_baseline(groups, data, chunk, budget_s=15.0):
                        f"{dt:.1f} s, oracle with unique=true semantics")
 
 
+def copy_calibration(nbytes, reps):
+    """Device-to-device copies of a known size: under rocprofv3 --pmc the
+    FETCH_SIZE / WRITE_SIZE of this copy kernel give the counters' scale."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    hip_sync()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def timed(eng, opts, steps, warmup):
     for _ in range(warmup):
         eng.run(opts)
@@ -117,9 +133,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["plain", "batched"], default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (counter calibration)")
     ap.add_argument("--oracle-check", action="store_true",
                     help="also compare reports with the oracle on three full chunks (slow)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration << 20, max(1, a.steps))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the (group, chunk) units would shard across GPUs with no exchange)"}))
