@@ -22,7 +22,7 @@ def test_literal64_full_size_bit_exact(engine):
     want, _ = oracle.run(groups, data, chunk, 1)
     engine.load_groups(groups, 3)
     engine.set_input(data, chunk, 1)
-    assert want.size > 1000
+    assert want.size > 20  # 4^-L per position for L = 8..16: tens of matches
     for sched in ("plain", "batched"):
         for d in (1, 2, 3):
             say("literal64 full", sched, d)
