@@ -65,8 +65,9 @@ def sample_stats(groups, data, chunk, depth, n_units=4, sample_len=16384, thread
     t = time.time()
     _, st = oracle.run(groups, sample, L, len(picks), threads=threads, memo_depth=depth)
     dt = time.time() - t
-    per_byte = oracle.algorithmic_bytes(st, sample.size) / sample.size
-    per_byte_ref = oracle.algorithmic_bytes_reference_model(st, sample.size) / sample.size
+    unit_bytes = sample.size * len(groups)
+    per_byte = oracle.algorithmic_bytes(st, unit_bytes) / sample.size
+    per_byte_ref = oracle.algorithmic_bytes_reference_model(st, unit_bytes) / sample.size
     return per_byte, per_byte_ref, st, sample.size, dt, len(picks)
 
 

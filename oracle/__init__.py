@@ -27,7 +27,7 @@ class OracleStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "tested_non_aas", "matched_aas", "matched_advance", "successors", "reports",
         "enabled_items", "matched_items", "memo_list_items", "expanded_items",
-        "expanded_tests")]
+        "expanded_tests", "root_positions")]
 
 
 _lib = None
@@ -84,19 +84,21 @@ def run(groups, data, chunk_len, num_seg=None, is_dup=False, threads=0, unit_mas
     return out, stats
 
 
-def algorithmic_bytes_reference_model(stats, total_input_bytes):
+def algorithmic_bytes_reference_model(stats, unit_bytes):
     """The reference algorithm's bytes (every all-input state tested at every
-    position): L + 4*T_ne + 4*M_aas + 8*M + 4*S + 8*R."""
-    return (total_input_bytes + 4 * stats["tested_non_aas"] + 4 * stats["matched_aas"]
+    position), summed over units: L + 4*T_ne + 4*M_aas + 8*M + 4*S + 8*R, where
+    unit_bytes = L summed over (group, chunk) units."""
+    return (unit_bytes + 4 * stats["tested_non_aas"] + 4 * stats["matched_aas"]
             + 8 * stats["matched_advance"] + 4 * stats["successors"] + 8 * stats["reports"])
 
 
-def algorithmic_bytes(stats, total_input_bytes):
+def algorithmic_bytes(stats, unit_bytes):
     """Bytes a prefix-memoized engine (depth k, the stats' memo_depth) must move
-    at minimum: per input byte 1 B input + 8 B memo index entry; 4 B per memo
-    list entry materialised; per expanded item one 16 B successor record and a
-    4 B class word per non-all-input successor; 8 B per report."""
-    return (9 * total_input_bytes + 4 * stats["memo_list_items"]
+    at minimum, summed over units: 1 B of input per (group, position); an 8 B
+    memo index entry per position where an all-input state matches; 4 B per
+    memo list entry materialised; per expanded item one 16 B successor record
+    and a 4 B class word per non-all-input successor; 8 B per report."""
+    return (unit_bytes + 8 * stats["root_positions"] + 4 * stats["memo_list_items"]
             + 16 * stats["expanded_items"] + 4 * stats["expanded_tests"]
             + 8 * stats["reports"])
 

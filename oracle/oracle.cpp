@@ -62,6 +62,7 @@ struct oracle_stats {
   uint64_t memo_list_items;  // materialised from a depth-k memo list
   uint64_t expanded_items;   // materialised items that advance
   uint64_t expanded_tests;   // their non-all-input successors
+  uint64_t root_positions;   // positions where at least one all-input state matched
 };
 
 // Runs the oracle.  `unit_mask` (nullable, length num_groups*num_seg, index
@@ -113,12 +114,16 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
       const uint8_t b = input[p];
       nxt.clear();
       st.enabled_items += cur.size();
+      bool root_here = false;
       for (const SD &vd : cur) {
         const int32_t v = vd.first, d = vd.second;
         if (!is_aas[v]) st.tested_non_aas++;
         if (!((g.sets[(size_t)v * 8 + b / 32] >> (b % 32)) & 1u)) continue;
         st.matched_items++;
-        if (is_aas[v]) st.matched_aas++;
+        if (is_aas[v]) {
+          st.matched_aas++;
+          root_here = true;
+        }
         if (g.attrs[v] & 2) {
           uint32_t id = (uint32_t)v | ((uint32_t)j << 22);
           rep.push_back(((uint64_t)id << 32) | (uint32_t)p);
@@ -133,9 +138,12 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
             st.expanded_tests += nonaas_deg[v];
           }
         }
+        // an all-input successor is re-enabled as a root (depth 0) below
         const int32_t dn = d + 1 < K ? d + 1 : K;
-        for (int e = g.row[v]; e < g.row[v + 1]; e++) nxt.emplace_back(g.col[e], dn);
+        for (int e = g.row[v]; e < g.row[v + 1]; e++)
+          if (!is_aas[g.col[e]]) nxt.emplace_back(g.col[e], dn);
       }
+      st.root_positions += root_here;
       for (int i = 0; i < g.num_aas; i++) nxt.emplace_back(g.aas[i], 0);
       merge(nxt);
       cur.swap(nxt);
@@ -162,6 +170,7 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
     tot.memo_list_items += per_stats[u].memo_list_items;
     tot.expanded_items += per_stats[u].expanded_items;
     tot.expanded_tests += per_stats[u].expanded_tests;
+    tot.root_positions += per_stats[u].root_positions;
   }
   // compareResult order: by offset (low word), then by id (high word)
   std::sort(all.begin(), all.end(), [](uint64_t a, uint64_t b) {

@@ -128,3 +128,19 @@ def test_memo_work_counts_on_a_closed_form(k):
     assert st["expanded_items"] == {1: 2 * n, 2: n, 3: 0}[k]
     assert st["expanded_tests"] == st["expanded_items"]
     assert st["reports"] == n
+    assert st["root_positions"] == n               # every 'a'
+
+
+def test_memo_depth_restarts_at_all_input_successors():
+    # all-input '*' with a self-loop feeding 'a' -> 'b' (report): the loop edge
+    # re-enters an all-input state, so chains never grow through it.
+    star = list(range(256))
+    g = make_group(3, [(0, 0), (0, 1), (1, 2)], [star, [ord("a")], [ord("b")]],
+                   all_input=[0], reporting=[2])
+    data = b"ab" * 30
+    _, st = oracle.run([g], data, len(data), 1, memo_depth=2)
+    assert st["root_positions"] == len(data)
+    # 'a' is enabled one position after a '*' match: every 'a' but the first
+    assert st["memo_list_items"] == 29            # each such 'a' at depth 1
+    assert st["expanded_items"] == 29             # each expands to 'b'
+    assert st["reports"] == 29
