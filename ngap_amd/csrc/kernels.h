@@ -23,6 +23,8 @@ struct DevImage {
   const int32_t *list[kMaxMemoDepth];
   const uint32_t *key[kMaxMemoDepth];
   const uint32_t *dir[kMaxMemoDepth];
+  const uint32_t *mask1;     // 8 words per group
+  const uint32_t *mask2;     // 2048 words per group (depth >= 2 built)
 };
 
 // One batched work item: roots at positions [begin, end) of chunk `chunk` of
@@ -64,7 +66,7 @@ constexpr uint32_t kBlockThreads = 256;
 
 // LDS bytes per wave for the given capacities
 NGAP_HD inline uint32_t wave_lds_bytes(uint32_t ring, uint32_t rbuf, uint32_t dc) {
-  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64) * 4 + 4;
+  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64 + 8) * 4 + 4;
 }
 
 }  // namespace ngapamd

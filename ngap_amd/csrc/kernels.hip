@@ -65,6 +65,7 @@ struct WaveState {
   uint32_t *sh_pref;  // 65
   uint32_t *sh_beg;   // 64
   uint32_t *sh_pos;   // 64
+  uint32_t *mask1;    // 8: bytes that start a root in the current group
   uint64_t *spill;
   uint32_t head, tail, rn, spill_n;
   uint32_t lane;
@@ -144,10 +145,11 @@ NGAP_DEV void refill_from_spill(const KParams &P, WaveState &W) {
   wsync();
 }
 
-// Processes a variable number of list entries per lane as one flat index
-// space (prefix sum over the wave), 64 entries per step.
-// MODE 0: emit each entry as a report; MODE 1: push each entry as an item.
-template <int MODE>
+// Processes a variable number of memo-list entries per lane as one flat index
+// space (prefix sum over the wave), 64 entries per step.  Every entry that
+// reports is emitted; entries are also pushed as items unless the lane's
+// list is report-only (bit 31 of beg), the state has no successors, or the
+// position is the chunk's last.
 NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg,
                       uint32_t pos, const int32_t *list) {
   uint32_t incl = wave_incl_scan(cnt, W.lane);
@@ -163,6 +165,7 @@ NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg
     bool ok = j < total;
     int32_t e = 0;
     uint32_t ppos = 0;
+    bool ronly = true;
     if (ok) {
       int lo = 0, hi = 64;  // largest o with pref[o] <= j (pref[64] = total > j)
       while (lo < hi) {
@@ -170,29 +173,26 @@ NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg
         if (W.sh_pref[mid] <= j) lo = mid;
         else hi = mid - 1;
       }
-      e = list[W.sh_beg[lo] + (j - W.sh_pref[lo])];
+      const uint32_t bg = W.sh_beg[lo];
+      ronly = bg >> 31;
+      e = list[(bg & 0x7fffffffu) + (j - W.sh_pref[lo])];
       ppos = W.sh_pos[lo];
     }
-    uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-    if (MODE == 0) {
-      emit(P, W, ok, ((uint64_t)ppos << 32) | u | W.gtag);
-    } else {
-      bool keep = ok && !(e & kFlagDeadEnd);
-      if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, ppos));
-      push(P, W, keep, item_of(u, ppos));
-    }
+    const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+    emit(P, W, ok && (e & kFlagReport), ((uint64_t)ppos << 32) | u | W.gtag);
+    bool keep = ok && !ronly && !(e & kFlagDeadEnd) && (ppos + 1 < W.cend);
+    if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, ppos));
+    push(P, W, keep, item_of(u, ppos));
   }
   wsync();
 }
 
-NGAP_DEV int32_t memo_lookup(const KParams &P, const WaveState &W, int d, uint32_t p) {
-  const uint8_t *in = P.input;
-  if (d == 1) return (int32_t)(W.g * 256u + in[p]);
-  uint32_t key = 0;
-  for (int i = 0; i < d; i++) key = (key << 8) | in[p + i];
-  uint32_t first = key >> (8 * (d - 1));
+// Index of the depth-d (2 or 3) list of a d-gram key, -1 if it has none:
+// the first byte selects a bucket of sorted keys, a binary search finds it.
+NGAP_DEV int32_t memo_lookup(const KParams &P, const WaveState &W, int d, uint32_t key) {
+  const uint32_t first = key >> (8 * (d - 1));
   const uint32_t *dir = P.img.dir[d - 1] + (size_t)W.g * 257;
-  uint32_t kb = d == 2 ? W.key_base2 : W.key_base3;
+  const uint32_t kb = d == 2 ? W.key_base2 : W.key_base3;
   int lo = (int)(kb + dir[first]), hi = (int)(kb + dir[first + 1]) - 1;
   const uint32_t *keys = P.img.key[d - 1];
   while (lo <= hi) {
@@ -205,28 +205,66 @@ NGAP_DEV int32_t memo_lookup(const KParams &P, const WaveState &W, int d, uint32
   return -1;
 }
 
-// Roots for positions [p0, pend): one lane per position.  A lane uses the
+// Roots for positions [p0, pend): one lane per position.  A position whose
+// byte starts no root costs one load and one bit test.  A lane uses the
 // deepest memo level that fits before the chunk end (kernel_ngap_OA.cu:243-
 // 244); shallower levels contribute their reports only.
 NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
-  uint32_t p = p0 + W.lane;
-  bool act = p < pend;
-  uint32_t left = act ? W.cend - p : 0;
-  uint32_t k = act ? (left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth) : 0;
-  for (int d = 1; d <= P.memo_depth; d++) {
-    bool use = act && (uint32_t)d <= k;
-    uint32_t beg = 0, len = 0, nr = 0;
-    if (use) {
-      int32_t li = memo_lookup(P, W, d, p);
+  const uint8_t *in = P.input;
+  const uint32_t p = p0 + W.lane;
+  const bool act = p < pend;
+  const uint32_t b0 = act ? in[p] : 0u;
+  bool has = act && ((W.mask1[b0 >> 5] >> (b0 & 31)) & 1u);
+  if (wballot(has) == 0) return;
+  const uint32_t left = act ? W.cend - p : 0u;
+  const uint32_t k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
+  const uint32_t kReportOnly = 1u << 31;
+  // depth 1: all-input states matching b0
+  {
+    uint32_t beg = 0, cnt = 0;
+    if (has) {
+      const uint32_t li = W.g * 256u + b0;
+      beg = P.img.beg[0][li];
+      cnt = k == 1 ? P.img.beg[0][li + 1] - beg : P.img.nrep[0][li];
+      if (k != 1) beg |= kReportOnly;
+    }
+    flatten(P, W, cnt, beg, p, P.img.list[0]);
+  }
+  if (P.memo_depth < 2) return;
+  // depth 2: the 2-gram filter, then the bucketed key index
+  bool has2 = has && k >= 2;
+  uint32_t key = 0;
+  if (has2) {
+    key = (b0 << 8) | in[p + 1];
+    has2 = (P.img.mask2[(size_t)W.g * 2048 + (key >> 5)] >> (key & 31)) & 1u;
+  }
+  if (wballot(has2) == 0) return;
+  {
+    uint32_t beg = 0, cnt = 0;
+    if (has2) {
+      const int32_t li = memo_lookup(P, W, 2, key);
       if (li >= 0) {
-        beg = P.img.beg[d - 1][li];
-        len = P.img.beg[d - 1][li + 1] - beg;
-        nr = P.img.nrep[d - 1][li];
+        beg = P.img.beg[1][li];
+        cnt = k == 2 ? P.img.beg[1][li + 1] - beg : P.img.nrep[1][li];
+        if (k != 2) beg |= kReportOnly;
+      } else {
+        has2 = false;
       }
     }
-    flatten<0>(P, W, nr, beg, p + d - 1, P.img.list[d - 1]);
-    bool items = use && (uint32_t)d == k && (p + d < W.cend);
-    flatten<1>(P, W, items ? len : 0, beg, p + d - 1, P.img.list[d - 1]);
+    flatten(P, W, cnt, beg, p + 1, P.img.list[1]);
+  }
+  if (P.memo_depth < 3) return;
+  bool has3 = has2 && k >= 3;
+  {
+    uint32_t beg = 0, cnt = 0;
+    if (has3) {
+      const int32_t li = memo_lookup(P, W, 3, (key << 8) | in[p + 2]);
+      if (li >= 0) {
+        beg = P.img.beg[2][li];
+        cnt = P.img.beg[2][li + 1] - beg;
+      }
+    }
+    flatten(P, W, cnt, beg, p + 2, P.img.list[2]);
   }
 }
 
@@ -305,6 +343,7 @@ NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t c)
   W.cstart = c * P.chunk_len;
   W.cend = W.cstart + P.chunk_len;
   for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
+  if (W.lane < 8) W.mask1[W.lane] = P.img.mask1[g * 8 + W.lane];
   wsync();
 }
 
@@ -316,6 +355,7 @@ NGAP_DEV void setup_wave(const KParams &P, WaveState &W, unsigned char *base,
   W.sh_pref = reinterpret_cast<uint32_t *>(W.dc + P.dc_cap);
   W.sh_beg = W.sh_pref + 65;
   W.sh_pos = W.sh_beg + 64;
+  W.mask1 = W.sh_pos + 64;
   W.spill = spill;
   W.head = W.tail = W.rn = W.spill_n = 0;
   W.lane = lane_id();

@@ -151,11 +151,16 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     for (int i = 0; i < hg.num_aas; i++)
       for (int b = 0; b < 256; b++)
         if (has_byte(hg.sets, hg.aas[i], b)) l1[b].push_back(hg.aas[i]);
+    size_t m1 = img.mask1.size();
+    img.mask1.resize(m1 + 8, 0u);
     for (int b = 0; b < 256; b++) {
       std::sort(l1[b].begin(), l1[b].end());
       l1[b].erase(std::unique(l1[b].begin(), l1[b].end()), l1[b].end());
+      if (!l1[b].empty()) img.mask1[m1 + b / 32] |= 1u << (b % 32);
       push_list(img.memo[0], l1[b], tag);
     }
+    size_t m2 = img.mask2.size();
+    if (memo_depth >= 2) img.mask2.resize(m2 + 2048, 0u);
     // deeper levels
     std::vector<char> mark(n, 0);
     std::vector<std::vector<int32_t>> bucket(256);
@@ -187,6 +192,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
           int first = (int)(key >> (8 * (d - 1)));
           while (last_first < first) m.dir[dir0 + (++last_first)] = (uint32_t)(m.key.size() - gm.memo_key_base[d - 2]);
           m.key.push_back(key);
+          if (d == 2) img.mask2[m2 + (key >> 5)] |= 1u << (key & 31);
           cur_lists.push_back(bucket[b]);
           cur_keys.push_back(key);
           push_list(m, bucket[b], tag);
@@ -206,6 +212,11 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     img.stat_selfloop.push_back(selfloop);
     img.total_states += n;
   }
+  for (auto &m : img.memo)
+    if (m.list.size() >= (1ull << 31)) {
+      err = "memo tables too large (2^31 entries); use a smaller precompute depth";
+      return false;
+    }
   if (img.succ4.size() / 4 >= (size_t)INT32_MAX / 2) {
     err = "automata image too large";
     return false;

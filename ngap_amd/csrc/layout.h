@@ -77,6 +77,10 @@ struct Image {
   std::vector<uint32_t> cls;    // 8 per state
   std::vector<int32_t> sod;     // tagged SOD ids
   MemoLevel memo[kMaxMemoDepth];// memo[0] = depth 1 (always built)
+  // root filters: per group 8 words (byte b has a depth-1 list) and, when
+  // depth >= 2 is built, 2048 words (2-gram key has a depth-2 list)
+  std::vector<uint32_t> mask1;
+  std::vector<uint32_t> mask2;
   // statistics (for the learned scheduler's features and reports)
   std::vector<uint32_t> stat_edges, stat_aas, stat_sod, stat_report,
       stat_wide, stat_selfloop;

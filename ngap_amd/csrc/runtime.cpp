@@ -89,13 +89,14 @@ struct ngap_engine {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Image img;
   bool loaded = false;
-  DevBuf d_meta, d_succ4, d_cls, d_sod;
+  DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1, d_mask2;
   DevBuf d_beg[kMaxMemoDepth], d_nrep[kMaxMemoDepth], d_list[kMaxMemoDepth],
       d_key[kMaxMemoDepth], d_dir[kMaxMemoDepth];
   DevBuf d_input;
   uint64_t chunk_len = 0;
   uint32_t num_seg = 0;
   bool have_input = false;
+  double byte_freq[256] = {0};   // input byte histogram (normalised)
   DevBuf d_reports, d_counters, d_overflow, d_spill, d_tasks, d_order, d_task_items;
   uint64_t report_cap = 0;
   uint64_t last_count = 0;
@@ -186,6 +187,8 @@ int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t nu
   HIP_TRY(eng->d_succ4.upload(im.succ4));
   HIP_TRY(eng->d_cls.upload(im.cls));
   HIP_TRY(eng->d_sod.upload(im.sod));
+  HIP_TRY(eng->d_mask1.upload(im.mask1));
+  HIP_TRY(eng->d_mask2.upload(im.mask2));
   for (int d = 0; d < kMaxMemoDepth; d++) {
     HIP_TRY(eng->d_beg[d].upload(im.memo[d].beg));
     HIP_TRY(eng->d_nrep[d].upload(im.memo[d].nrep));
@@ -213,6 +216,9 @@ int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
   size_t n = (size_t)chunk_len * num_seg;
   HIP_TRY(eng->d_input.reserve(n + 16));
   HIP_TRY(hipMemcpy(eng->d_input.p, data, n, hipMemcpyHostToDevice));
+  uint64_t hist[256] = {0};
+  for (size_t i = 0; i < n; i++) hist[data[i]]++;
+  for (int b = 0; b < 256; b++) eng->byte_freq[b] = (double)hist[b] / (double)n;
   eng->chunk_len = chunk_len;
   eng->num_seg = num_seg;
   eng->have_input = true;
@@ -302,6 +308,8 @@ static int run_impl(ngap_engine *eng, const ngap_run_opts &o, ngap_run_stats *st
     P.img.key[d] = eng->d_key[d].as<uint32_t>();
     P.img.dir[d] = eng->d_dir[d].as<uint32_t>();
   }
+  P.img.mask1 = eng->d_mask1.as<uint32_t>();
+  P.img.mask2 = eng->d_mask2.as<uint32_t>();
   P.input = eng->d_input.as<uint8_t>();
   P.chunk_len = (uint32_t)eng->chunk_len;
   P.num_seg = S;
@@ -338,28 +346,47 @@ static int run_impl(ngap_engine *eng, const ngap_run_opts &o, ngap_run_stats *st
     P.spill_cap = spill;
     HIP_TRY(eng->d_spill.reserve((size_t)units * P.waves_per_block * spill * 8));
   } else {
+    // Static schedule: a group whose roots are rare under the input's byte
+    // histogram takes whole chunks per work item; others take `window`
+    // positions.  Work items run longest-expected-first.
     uint32_t win = o.window ? o.window : 4096;
     if (win > P.chunk_len) win = P.chunk_len;
+    std::vector<double> root_p(G, 0.0);
+    for (uint32_t g = 0; g < G; g++)
+      for (int b = 0; b < 256; b++)
+        if ((im.mask1[g * 8 + b / 32] >> (b % 32)) & 1u) root_p[g] += eng->byte_freq[b];
+    std::vector<double> cost;
     for (uint32_t c = 0; c < S; c++)
-      for (uint32_t g = 0; g < G; g++)
-        for (uint32_t a = 0; a < P.chunk_len; a += win) {
-          uint32_t b = std::min<uint64_t>((uint64_t)a + win, P.chunk_len);
+      for (uint32_t g = 0; g < G; g++) {
+        uint32_t wg = root_p[g] < 1.0 / 16 ? P.chunk_len : win;
+        for (uint32_t a = 0; a < P.chunk_len; a += wg) {
+          uint32_t b = std::min<uint64_t>((uint64_t)a + wg, P.chunk_len);
           tasks.push_back({g, c, c * P.chunk_len + a, c * P.chunk_len + b});
+          cost.push_back((b - a) * (0.02 + root_p[g]));
         }
+      }
+    std::vector<uint32_t> lpt(tasks.size());
+    for (uint32_t i = 0; i < lpt.size(); i++) lpt[i] = i;
+    std::stable_sort(lpt.begin(), lpt.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
     if (tasks.size() >= (1u << 31)) return fail(NGAP_ERR_INVALID, "too many work items");
     n_flags = (uint32_t)tasks.size();
     HIP_TRY(eng->d_tasks.upload(tasks));
     P.tasks = eng->d_tasks.as<Task>();
     P.num_tasks = (uint32_t)tasks.size();
+    std::vector<uint32_t> ord = lpt;
     if (o.task_order) {
       if (o.task_order_len != tasks.size())
         return fail(NGAP_ERR_INVALID, "task_order length does not match the work items");
-      std::vector<uint32_t> ord(o.task_order, o.task_order + o.task_order_len);
-      for (uint32_t x : ord)
-        if (x >= tasks.size()) return fail(NGAP_ERR_INVALID, "task_order entry out of range");
-      HIP_TRY(eng->d_order.upload(ord));
-      P.task_order = eng->d_order.as<uint32_t>();
+      ord.assign(o.task_order, o.task_order + o.task_order_len);
+      std::vector<char> seen(tasks.size(), 0);
+      for (uint32_t x : ord) {
+        if (x >= tasks.size() || seen[x])
+          return fail(NGAP_ERR_INVALID, "task_order must be a permutation of the work items");
+        seen[x] = 1;
+      }
     }
+    HIP_TRY(eng->d_order.upload(ord));
+    P.task_order = eng->d_order.as<uint32_t>();
     HIP_TRY(eng->d_task_items.reserve(tasks.size() * 8));
     P.task_items = eng->d_task_items.as<unsigned long long>();
     int per_cu = 0;
