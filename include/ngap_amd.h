@@ -55,6 +55,14 @@ extern "C" {
 
 typedef struct ngap_engine ngap_engine;
 
+/* One batched work item: roots at positions [begin, end) of chunk `chunk`
+ * (offsets relative to the chunk start) for group `group`; its items run to
+ * the end of the chunk.  The work items of one (group, chunk) unit must tile
+ * [0, chunk_len) exactly. */
+typedef struct {
+  uint32_t group, chunk, begin, end;
+} ngap_task;
+
 /* One group as Graph/Csr hold it (code/include/commons/graph.h:101-248). */
 typedef struct {
   int32_t num_states;
@@ -86,8 +94,11 @@ typedef struct {
   uint32_t spill_capacity;      /* items per wave beyond the LDS ring, 0 = auto */
   uint32_t persistent_blocks;   /* batched: workgroups, 0 = auto */
   int32_t use_graph;            /* batched: replay a captured hipGraph */
-  const uint32_t *task_order;   /* batched: optional work-item order (host) */
-  uint32_t task_order_len;
+  const ngap_task *tasks;       /* batched: work items in run order (NULL = the
+                                   engine's static schedule) */
+  uint32_t num_tasks;
+  int32_t fallback;             /* re-run overflowing work items with a larger
+                                   worklist instead of failing */
 } ngap_run_opts;
 
 typedef struct {
@@ -97,6 +108,8 @@ typedef struct {
   uint64_t items;               /* (state, offset) items expanded */
   uint32_t num_units;           /* (group, chunk) pairs */
   uint32_t num_tasks;           /* batched work items */
+  uint32_t fallback_tasks;      /* work items re-run after an overflow */
+  double fallback_ms;           /* device time of those re-runs */
 } ngap_run_stats;
 
 const char *ngap_last_error(void);
@@ -120,8 +133,37 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats)
  * most cap entries are written (out may be NULL to query the count). */
 int ngap_get_reports(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out);
 
-/* Per-work-item item counts of the last batched run (frontier traces). */
+/* Per-work-item item counts of the last batched run (frontier traces), in
+ * the order of the work items. */
 int ngap_get_task_items(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out);
+/* The work items of the last batched run (the static schedule's choice when
+ * none were given). */
+int ngap_get_tasks(ngap_engine *eng, ngap_task *out, uint64_t cap, uint64_t *n_out);
+
+/* ---- serving: continuous batching of scan requests on resident groups ----
+ * No counterpart in the reference (one input per process there).  A request
+ * (group, bytes) is scanned as one chunk.  Requests queue; whenever one of
+ * max_inflight launch slots is free and the HBM budget allows, all queued
+ * requests that fit are packed into one launch on that slot's stream. */
+typedef struct ngap_server ngap_server;
+
+/* opts: algorithm / precompute depth / unique as for ngap_run.  budget_bytes
+ * 0 = 200 GB. */
+int ngap_server_create(ngap_engine *eng, uint64_t budget_bytes, int32_t max_inflight,
+                       const ngap_run_opts *opts, ngap_server **out);
+int ngap_server_destroy(ngap_server *srv);
+int ngap_server_submit(ngap_server *srv, int32_t group, const uint8_t *data, uint64_t len,
+                       uint64_t *req_id);
+/* Retires finished launches and starts new ones; block != 0 waits for at
+ * least one launch to finish when none has. */
+int ngap_server_poll(ngap_server *srv, int32_t block);
+/* state: 0 queued, 1 in flight, 2 done, 3 failed.  When done, the request's
+ * sorted unique reports (offsets relative to the request) are copied to out;
+ * a request is forgotten once all of them were delivered.  latency_ms is
+ * submit-to-completion wall time. */
+int ngap_server_take(ngap_server *srv, uint64_t req_id, int32_t *state, uint64_t *out,
+                     uint64_t cap, uint64_t *n_out, double *latency_ms);
+int ngap_server_pending(ngap_server *srv, uint64_t *queued, uint64_t *inflight);
 
 /* ---- front-end: ANML -> groups in the reference state-id namespace ---- */
 typedef struct ngap_frontend ngap_frontend;

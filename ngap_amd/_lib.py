@@ -43,6 +43,11 @@ class GroupDesc(C.Structure):
     ]
 
 
+class Task(C.Structure):
+    _fields_ = [("group", C.c_uint32), ("chunk", C.c_uint32), ("begin", C.c_uint32),
+                ("end", C.c_uint32)]
+
+
 class RunOpts(C.Structure):
     _fields_ = [
         ("algorithm", C.c_int32),
@@ -60,8 +65,9 @@ class RunOpts(C.Structure):
         ("spill_capacity", C.c_uint32),
         ("persistent_blocks", C.c_uint32),
         ("use_graph", C.c_int32),
-        ("task_order", C.c_void_p),
-        ("task_order_len", C.c_uint32),
+        ("tasks", C.c_void_p),
+        ("num_tasks", C.c_uint32),
+        ("fallback", C.c_int32),
     ]
 
 
@@ -73,6 +79,8 @@ class RunStats(C.Structure):
         ("items", C.c_uint64),
         ("num_units", C.c_uint32),
         ("num_tasks", C.c_uint32),
+        ("fallback_tasks", C.c_uint32),
+        ("fallback_ms", C.c_double),
     ]
 
 
@@ -87,6 +95,16 @@ SIGNATURES = {
     "ngap_run": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     "ngap_get_reports": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "ngap_get_task_items": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "ngap_get_tasks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "ngap_server_create": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(RunOpts),
+                                     C.POINTER(C.c_void_p)]),
+    "ngap_server_destroy": (C.c_int, [C.c_void_p]),
+    "ngap_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64,
+                                     C.POINTER(C.c_uint64)]),
+    "ngap_server_poll": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ngap_server_take": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32), C.c_void_p,
+                                   C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "ngap_server_pending": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ngap_frontend_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                      C.POINTER(C.c_void_p)]),
     "ngap_frontend_load_text": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

@@ -27,11 +27,13 @@ struct DevImage {
   const uint32_t *mask2;     // 2048 words per group (depth >= 2 built)
 };
 
-// One batched work item: roots at positions [begin, end) of chunk `chunk` of
-// group `group`; descendants run to the end of the chunk.
+// One batched work item: roots at positions [begin, end) of the chunk
+// [cstart, cend) scanned by group `group`; descendants run to cend.  Bit 0 of
+// flags seeds the group's start-of-data states at cstart.
 struct Task {
-  uint32_t group, chunk, begin, end;
+  uint32_t group, cstart, cend, begin, end, flags;
 };
+constexpr uint32_t kTaskSeedStart = 1;
 
 struct KParams {
   DevImage img;

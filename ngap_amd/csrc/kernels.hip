@@ -332,7 +332,8 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
   }
 }
 
-NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t c) {
+NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t cstart,
+                         uint32_t cend) {
   const GroupMeta gm = P.img.meta[g];
   W.g = g;
   W.gtag = g << 22;
@@ -340,8 +341,8 @@ NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t c)
   W.cls_base = gm.cls_base;
   W.key_base2 = gm.memo_key_base[0];
   W.key_base3 = gm.memo_key_base[1];
-  W.cstart = c * P.chunk_len;
-  W.cend = W.cstart + P.chunk_len;
+  W.cstart = cstart;
+  W.cend = cend;
   for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
   if (W.lane < 8) W.mask1[W.lane] = P.img.mask1[g * 8 + W.lane];
   wsync();
@@ -443,7 +444,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_plain(KParams P) {
   WaveState W;
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)unit * P.waves_per_block + wave) * P.spill_cap);
-  begin_unit(P, W, g, c);
+  begin_unit(P, W, g, c * P.chunk_len, (c + 1) * P.chunk_len);
   if (threadIdx.x == 0) *cursor = W.cstart;
   __syncthreads();
   if (wave == 0 && (c == 0 || P.is_dup)) {
@@ -473,10 +474,10 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
     if (t >= P.num_tasks) break;
     const uint32_t ti = P.task_order ? P.task_order[t] : t;
     const Task T = P.tasks[ti];
-    begin_unit(P, W, T.group, T.chunk);
+    begin_unit(P, W, T.group, T.cstart, T.cend);
     W.overflow = false;
     const unsigned long long items0 = W.items;
-    if (T.begin == W.cstart && (T.chunk == 0 || P.is_dup)) {
+    if (T.flags & kTaskSeedStart) {
       const GroupMeta gm = P.img.meta[T.group];
       seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
     }
@@ -491,12 +492,14 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
 
-// Clears the per-run counters with a kernel (graph-capture friendly).
+// Clears the per-run counters with a kernel (graph-capture friendly).  A
+// re-run of overflowed work items keeps the report count (keep_reports).
 __global__ void k_reset(unsigned long long *report_count, unsigned long long *items,
-                        uint32_t *task_head, uint32_t *overflow, uint32_t n_overflow) {
+                        uint32_t *task_head, uint32_t *overflow, uint32_t n_overflow,
+                        int keep_reports) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
-    *report_count = 0;
+    if (!keep_reports) *report_count = 0;
     *items = 0;
     *task_head = 0;
   }
@@ -505,11 +508,12 @@ __global__ void k_reset(unsigned long long *report_count, unsigned long long *it
 
 // Host-side launch wrappers (this translation unit owns the kernels).
 hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
-                        uint32_t *head, uint32_t *ovf, uint32_t n) {
+                        uint32_t *head, uint32_t *ovf, uint32_t n, int keep_reports) {
   uint32_t blocks = (n + 255) / 256;
   if (blocks == 0) blocks = 1;
   if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(256), 0, s, rc, items, head, ovf, n);
+  hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(256), 0, s, rc, items, head, ovf, n,
+                     keep_reports);
   return hipGetLastError();
 }
 

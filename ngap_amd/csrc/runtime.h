@@ -1,0 +1,104 @@
+// Host-side engine state shared by the C ABI entry points (runtime.cpp) and
+// the request server (server.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ngap_amd.h"
+#include "kernels.h"
+#include "layout.h"
+
+void ngap_set_last_error_internal(const std::string &m);  // errors.cpp
+int ngap_fail(int code, const std::string &msg);
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return ngap_fail(_e == hipErrorOutOfMemory ? NGAP_ERR_OOM : NGAP_ERR_DEVICE,    \
+                       std::string(#expr) + ": " + hipGetErrorString(_e));            \
+  } while (0)
+
+namespace ngapamd {
+
+// LDS capacities per wave (entries)
+constexpr uint32_t kRing = 512, kRbuf = 128, kDc = 256;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release();
+  hipError_t reserve(size_t bytes);  // grows; contents not preserved
+  template <class T>
+  hipError_t upload(const std::vector<T> &v) {
+    hipError_t e = reserve(v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (!v.empty()) e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+  }
+  template <class T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// One stream with its own per-run buffers and captured graph.
+struct RunCtx {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf reports, counters, overflow, spill, tasks, task_items, fb_tasks, fb_spill;
+  uint64_t report_cap = 0, last_count = 0;
+  bool last_report_off = false, have_run = false;
+  std::vector<Task> last_tasks;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;
+  RunCtx() = default;
+  RunCtx(const RunCtx &) = delete;
+  ~RunCtx();
+  int init();
+  void drop_graph();
+};
+
+struct Mapping {
+  int memo_depth;
+  int priv_limit;
+};
+
+bool map_algorithm(int algorithm, int depth_req, int loaded_depth, Mapping &m, std::string &err);
+uint32_t lds_bytes();
+
+}  // namespace ngapamd
+
+struct ngap_engine {
+  int device = 0;
+  int num_cus = 0;
+  ngapamd::Image img;
+  bool loaded = false;
+  ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1, d_mask2;
+  ngapamd::DevBuf d_beg[ngapamd::kMaxMemoDepth], d_nrep[ngapamd::kMaxMemoDepth],
+      d_list[ngapamd::kMaxMemoDepth], d_key[ngapamd::kMaxMemoDepth],
+      d_dir[ngapamd::kMaxMemoDepth];
+  ngapamd::DevBuf d_input;
+  uint64_t chunk_len = 0;
+  uint32_t num_seg = 0;
+  bool have_input = false;
+  double byte_freq[256] = {0};  // input byte histogram (normalised)
+  ngapamd::RunCtx ctx;          // the context ngap_run uses
+};
+
+namespace ngapamd {
+KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_opts &o);
+void attach_ctx(KParams &P, RunCtx &ctx);
+int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks);
+// Runs tasks on ctx (batched kernel), with the overflow fallback of o.
+int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> &tasks,
+              const ngap_run_opts &o, ngap_run_stats *st);
+// Device reports of ctx's last run, sorted and unique (device encoding).
+int sorted_reports(RunCtx &ctx, std::vector<uint64_t> &out);
+}  // namespace ngapamd

@@ -1,0 +1,312 @@
+// Request server: continuous batching of scan requests on resident groups.
+//
+// A request is (group, bytes), scanned as one chunk (start-of-data states at
+// its first byte).  Requests wait in a queue; whenever one of `max_inflight`
+// slots is free and the HBM budget allows, every waiting request that fits is
+// packed into one persistent launch on that slot's stream (work items of 4 KiB
+// windows, each request its own chunk bounds), so launches overlap on the
+// device while new requests keep arriving.  Finished launches are retired by
+// polling their events: reports are split back per request by offset range.
+// There is no counterpart in the reference (it runs one input per process).
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <memory>
+#include <unordered_map>
+
+#include "runtime.h"
+
+namespace ngapamd {
+hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
+                        uint32_t *head, uint32_t *ovf, uint32_t n, int keep_reports);
+hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
+}  // namespace ngapamd
+
+using namespace ngapamd;
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  int32_t group = 0;
+  std::vector<uint8_t> data;
+  double t_submit = 0, t_done = 0;
+  int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed
+  std::vector<uint64_t> reports;
+};
+
+struct Slot {
+  RunCtx ctx;
+  DevBuf input;
+  std::vector<uint8_t> host;    // staging copy, alive until the launch retires
+  bool busy = false;
+  std::vector<uint64_t> ids;
+  std::vector<uint32_t> base;   // request start offsets in the slot input
+  std::vector<Task> tasks;
+  uint64_t bytes = 0;           // HBM held while in flight
+  KParams P;
+  uint32_t blocks = 0;
+};
+
+constexpr uint32_t kServeWindow = 4096;
+
+}  // namespace
+
+struct ngap_server {
+  ngap_engine *eng = nullptr;
+  uint64_t budget = 0;
+  ngap_run_opts opts;
+  Mapping mp;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::deque<uint64_t> queue;
+  std::unordered_map<uint64_t, Request> reqs;
+  uint64_t next_id = 1;
+  uint64_t inflight_bytes = 0;
+  uint32_t blocks_per_launch = 0;
+};
+
+namespace {
+
+int launch_slot(ngap_server *s, Slot &sl) {
+  // pack queued requests that fit the budget
+  sl.ids.clear();
+  sl.base.clear();
+  sl.tasks.clear();
+  uint64_t total = 0;
+  const uint64_t spill_bytes = (uint64_t)s->blocks_per_launch * 4 * 65536 * 8;
+  while (!s->queue.empty()) {
+    const uint64_t id = s->queue.front();
+    const Request &r = s->reqs[id];
+    const uint64_t need = total + r.data.size();
+    const uint64_t hbm = need + need * 16 + spill_bytes;  // input + reports + spill
+    if (!sl.ids.empty() && (s->inflight_bytes + hbm > s->budget || need >= (1ull << 31))) break;
+    sl.ids.push_back(id);
+    sl.base.push_back((uint32_t)total);
+    total = need;
+    s->queue.pop_front();
+  }
+  if (sl.ids.empty()) return NGAP_OK;
+  std::vector<uint8_t> &buf = sl.host;
+  buf.resize(total);
+  for (size_t i = 0; i < sl.ids.size(); i++) {
+    Request &r = s->reqs[sl.ids[i]];
+    std::copy(r.data.begin(), r.data.end(), buf.begin() + sl.base[i]);
+    const uint32_t cs = sl.base[i], ce = cs + (uint32_t)r.data.size();
+    for (uint32_t a = cs; a < ce; a += kServeWindow) {
+      const uint32_t b = std::min<uint32_t>(a + kServeWindow, ce);
+      sl.tasks.push_back({(uint32_t)r.group, cs, ce, a, b, a == cs ? kTaskSeedStart : 0u});
+    }
+    r.state = 1;
+  }
+  // longest requests first inside the launch
+  std::stable_sort(sl.tasks.begin(), sl.tasks.end(), [](const Task &x, const Task &y) {
+    return (x.cend - x.cstart) > (y.cend - y.cstart);
+  });
+  const uint64_t report_cap = std::max<uint64_t>(1 << 20, total * 2);
+  HIP_TRY(sl.input.reserve(total + 16));
+  HIP_TRY(hipMemcpyAsync(sl.input.p, buf.data(), total, hipMemcpyHostToDevice, sl.ctx.stream));
+  HIP_TRY(sl.ctx.tasks.upload(sl.tasks));
+  HIP_TRY(sl.ctx.overflow.reserve(sl.tasks.size() * 4 + 4));
+  HIP_TRY(sl.ctx.reports.reserve(report_cap * 8));
+  sl.ctx.report_cap = report_cap;
+  HIP_TRY(sl.ctx.spill.reserve(spill_bytes));
+  KParams P = base_params(s->eng, s->mp, s->opts);
+  P.input = sl.input.as<uint8_t>();
+  P.chunk_len = 0;
+  P.num_seg = 0;
+  P.tasks = sl.ctx.tasks.as<Task>();
+  P.num_tasks = (uint32_t)sl.tasks.size();
+  P.overflow = sl.ctx.overflow.as<uint32_t>();
+  P.spill = sl.ctx.spill.as<uint64_t>();
+  P.spill_cap = 65536;
+  attach_ctx(P, sl.ctx);
+  sl.P = P;
+  sl.blocks = s->blocks_per_launch;
+  HIP_TRY(launch_reset(sl.ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
+                       P.num_tasks, 0));
+  HIP_TRY(launch_batched(sl.ctx.stream, P, sl.blocks, lds_bytes()));
+  HIP_TRY(hipEventRecord(sl.ctx.ev1, sl.ctx.stream));
+  sl.bytes = total + report_cap * 8 + spill_bytes;
+  s->inflight_bytes += sl.bytes;
+  sl.busy = true;
+  return NGAP_OK;
+}
+
+int retire_slot(ngap_server *s, Slot &sl) {
+  HIP_TRY(hipEventSynchronize(sl.ctx.ev1));
+  const uint32_t T = sl.P.num_tasks;
+  std::vector<uint32_t> flags(T);
+  HIP_TRY(hipMemcpy(flags.data(), sl.P.overflow, T * 4ull, hipMemcpyDeviceToHost));
+  unsigned long long cnt = 0;
+  HIP_TRY(hipMemcpy(&cnt, sl.ctx.counters.p, 8, hipMemcpyDeviceToHost));
+  bool redo = cnt > sl.ctx.report_cap;
+  for (uint32_t f : flags) redo |= f != 0;
+  if (redo) {
+    // overflow: re-run the whole launch synchronously with room for every
+    // report and the engine's overflow fallback for the worklists
+    KParams P = sl.P;
+    ngap_run_opts o = s->opts;
+    o.fallback = 1;
+    if (cnt > sl.ctx.report_cap) {
+      HIP_TRY(sl.ctx.reports.reserve(cnt * 8 + 8));
+      sl.ctx.report_cap = cnt + 1;
+    }
+    int rc = run_tasks(s->eng, sl.ctx, P, sl.tasks, o, nullptr);
+    if (rc) {
+      for (uint64_t id : sl.ids) s->reqs[id].state = 3;
+      sl.busy = false;
+      s->inflight_bytes -= sl.bytes;
+      return rc;
+    }
+  } else {
+    sl.ctx.last_count = cnt;
+  }
+  std::vector<uint64_t> v;
+  int rc = sorted_reports(sl.ctx, v);
+  if (rc) return rc;
+  // split by request: device reports are (offset << 32) | id, offsets global
+  // in the slot input; requests are contiguous and sorted by base
+  std::vector<size_t> order(sl.ids.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return sl.base[a] < sl.base[b]; });
+  const double t = now_ms();
+  size_t k = 0;
+  for (size_t oi = 0; oi < order.size(); oi++) {
+    const size_t i = order[oi];
+    Request &r = s->reqs[sl.ids[i]];
+    const uint64_t lo = sl.base[i], hi = lo + r.data.size();
+    r.reports.clear();
+    while (k < v.size() && (v[k] >> 32) < hi) {
+      const uint64_t off = (v[k] >> 32) - lo;
+      r.reports.push_back(((v[k] & 0xffffffffull) << 32) | off);
+      k++;
+    }
+    r.state = 2;
+    r.t_done = t;
+    std::vector<uint8_t>().swap(r.data);
+  }
+  sl.busy = false;
+  s->inflight_bytes -= sl.bytes;
+  return NGAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
+                       const ngap_run_opts *opts, ngap_server **out) {
+  if (!eng || !out || !opts) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
+  if (max_inflight < 1 || max_inflight > 16)
+    return ngap_fail(NGAP_ERR_INVALID, "max_inflight must be in [1, 16]");
+  std::unique_ptr<ngap_server> s(new ngap_server());
+  s->eng = eng;
+  s->budget = budget ? budget : (200ull << 30);
+  s->opts = *opts;
+  std::string err;
+  if (!map_algorithm(opts->algorithm, opts->precompute_depth, eng->img.memo_depth, s->mp, err))
+    return ngap_fail(NGAP_ERR_INVALID, err);
+  HIP_TRY(hipSetDevice(eng->device));
+  uint32_t blocks = 0;
+  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks);
+  if (rc) return rc;
+  s->blocks_per_launch = std::max<uint32_t>(1, blocks / (uint32_t)max_inflight);
+  for (int i = 0; i < max_inflight; i++) {
+    s->slots.emplace_back(new Slot());
+    rc = s->slots.back()->ctx.init();
+    if (rc) return rc;
+  }
+  *out = s.release();
+  return NGAP_OK;
+}
+
+int ngap_server_destroy(ngap_server *s) {
+  if (!s) return NGAP_OK;
+  for (auto &sl : s->slots)
+    if (sl->busy) (void)hipEventSynchronize(sl->ctx.ev1);
+  delete s;
+  return NGAP_OK;
+}
+
+int ngap_server_submit(ngap_server *s, int32_t group, const uint8_t *data, uint64_t len,
+                       uint64_t *req_id) {
+  if (!s || !req_id || (!data && len)) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  if (group < 0 || group >= s->eng->img.num_groups)
+    return ngap_fail(NGAP_ERR_INVALID, "group out of range");
+  if (len == 0 || len >= (1ull << 31)) return ngap_fail(NGAP_ERR_INVALID, "bad request length");
+  const uint64_t id = s->next_id++;
+  Request &r = s->reqs[id];
+  r.group = group;
+  r.data.assign(data, data + len);
+  r.t_submit = now_ms();
+  s->queue.push_back(id);
+  *req_id = id;
+  return NGAP_OK;
+}
+
+int ngap_server_poll(ngap_server *s, int32_t block) {
+  if (!s) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  HIP_TRY(hipSetDevice(s->eng->device));
+  bool retired = false;
+  for (auto &sl : s->slots) {
+    if (!sl->busy) continue;
+    hipError_t q = hipEventQuery(sl->ctx.ev1);
+    if (q == hipSuccess) {
+      int rc = retire_slot(s, *sl);
+      if (rc) return rc;
+      retired = true;
+    } else if (q != hipErrorNotReady) {
+      HIP_TRY(q);
+    }
+  }
+  for (auto &sl : s->slots) {
+    if (s->queue.empty()) break;
+    if (sl->busy) continue;
+    int rc = launch_slot(s, *sl);
+    if (rc) return rc;
+  }
+  if (block && !retired) {
+    // wait for the oldest launch in flight
+    for (auto &sl : s->slots)
+      if (sl->busy) {
+        int rc = retire_slot(s, *sl);
+        if (rc) return rc;
+        break;
+      }
+  }
+  return NGAP_OK;
+}
+
+int ngap_server_take(ngap_server *s, uint64_t req_id, int32_t *state, uint64_t *out,
+                     uint64_t cap, uint64_t *n_out, double *latency_ms) {
+  if (!s || !state || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  auto it = s->reqs.find(req_id);
+  if (it == s->reqs.end()) return ngap_fail(NGAP_ERR_INVALID, "unknown request");
+  Request &r = it->second;
+  *state = r.state;
+  *n_out = r.reports.size();
+  if (latency_ms) *latency_ms = r.state >= 2 ? r.t_done - r.t_submit : -1.0;
+  if (r.state == 2 && out) {
+    const uint64_t n = std::min<uint64_t>(cap, r.reports.size());
+    std::copy(r.reports.begin(), r.reports.begin() + n, out);
+    if (n == r.reports.size()) s->reqs.erase(it);  // delivered
+  }
+  return NGAP_OK;
+}
+
+int ngap_server_pending(ngap_server *s, uint64_t *queued, uint64_t *inflight) {
+  if (!s || !queued || !inflight) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  *queued = s->queue.size();
+  uint64_t n = 0;
+  for (auto &sl : s->slots)
+    if (sl->busy) n += sl->ids.size();
+  *inflight = n;
+  return NGAP_OK;
+}
+
+}  // extern "C"

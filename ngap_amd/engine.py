@@ -61,7 +61,7 @@ class Engine:
                 unique=False, is_dup=False, active_threshold=20, report_off=False,
                 result_capacity=54619400, window=0, spill_capacity=0, persistent_blocks=0,
                 use_graph=False, data_buffer_fetch_size=128, add_aas_start=0,
-                add_aas_interval=1, task_order=None):
+                add_aas_interval=1, tasks=None, fallback=False):
         o = _lib.RunOpts()
         _lib.lib().ngap_default_opts(C.byref(o))
         if algorithm not in ALGORITHMS:
@@ -81,10 +81,11 @@ class Engine:
         o.data_buffer_fetch_size = int(data_buffer_fetch_size)
         o.add_aas_start = int(add_aas_start)
         o.add_aas_interval = int(add_aas_interval)
-        if task_order is not None:
-            arr = np.ascontiguousarray(task_order, np.uint32)
-            o.task_order = arr.ctypes.data
-            o.task_order_len = arr.size
+        o.fallback = int(bool(fallback))
+        if tasks is not None:
+            arr = np.ascontiguousarray(np.asarray(tasks, np.uint32).reshape(-1, 4))
+            o.tasks = arr.ctypes.data
+            o.num_tasks = arr.shape[0]
             o._keep = arr  # keep the array alive with the options
         return o
 
@@ -97,7 +98,8 @@ class Engine:
         code = _lib.lib().ngap_run(self._h, C.byref(opts), C.byref(st))
         stats = dict(kernel_ms=st.kernel_ms, reports_raw=st.reports_raw,
                      overflow_units=st.overflow_units, items=st.items,
-                     num_units=st.num_units, num_tasks=st.num_tasks)
+                     num_units=st.num_units, num_tasks=st.num_tasks,
+                     fallback_tasks=st.fallback_tasks, fallback_ms=st.fallback_ms)
         if code != _lib.NGAP_OK:
             err = NgapError(code, _lib.lib().ngap_last_error().decode(errors="replace"))
             err.stats = stats
@@ -111,6 +113,16 @@ class Engine:
         out = np.zeros(n.value, np.uint64)
         if n.value:
             check(_lib.lib().ngap_get_reports(self._h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+    def tasks(self):
+        """Work items of the last batched run: uint32 [n, 4] (group, chunk,
+        begin, end), begin/end relative to the chunk start, in run order."""
+        n = C.c_uint64()
+        check(_lib.lib().ngap_get_tasks(self._h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 4), np.uint32)
+        if n.value:
+            check(_lib.lib().ngap_get_tasks(self._h, out.ctypes.data, n.value, C.byref(n)))
         return out
 
     def task_items(self):

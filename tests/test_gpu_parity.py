@@ -152,15 +152,39 @@ def test_chunk_of_one_byte_and_no_reports(engine):
 
 
 def test_overflow_is_reported_not_wrong(engine):
-    rng = np.random.default_rng(5)
-    g = synth.regex_group(rng, 5000, dotstar=False)
-    data = synth.zipf_bytes(rng, 1 << 16, 0.6)
+    # every byte reports: 4096 reports against a capacity of 10
+    g = make_group(1, [], [[ord("a")]], all_input=[0], reporting=[0])
     engine.load_groups([g], 2)
-    engine.set_input(data, 1 << 16, 1)
+    engine.set_input(b"a" * 4096, 4096, 1)
     say("tiny result capacity")
     with pytest.raises(NgapError) as ei:
-        engine.run(algorithm="nonblockingallgroups", precompute_depth=2, result_capacity=1)
+        engine.run(algorithm="nonblockingallgroups", precompute_depth=2, result_capacity=10)
     assert ei.value.code == 4
+    engine.run(algorithm="nonblockingallgroups", precompute_depth=2)
+    assert engine.reports().size == 4096
+
+
+def test_spill_overflow_is_reported(engine):
+    # a 4-ary tree of 'a' states three levels deep below an all-input root:
+    # every position creates 4 + 16 + 64 items, far beyond a 512-item ring when
+    # the spill capacity is forced to 64 items per wave.
+    n = 1 + 4 + 16 + 64
+    edges = [(0, 1 + i) for i in range(4)] + [(1 + i, 5 + 4 * i + j) for i in range(4) for j in range(4)]
+    edges += [(5 + i, 21 + 4 * i + j) for i in range(16) for j in range(4)]
+    g = make_group(n, edges, [[ord("a")]] * n, all_input=[0], reporting=list(range(21, n)))
+    data = b"a" * 20000
+    want, _ = oracle.run([g], data, len(data), 1)
+    engine.load_groups([g], 1)
+    engine.set_input(data, len(data), 1)
+    for sched in ("plain", "batched"):
+        say("spill overflow", sched)
+        with pytest.raises(NgapError) as ei:
+            engine.run(algorithm="nonblockinggroups", schedule=sched, spill_capacity=64,
+                       window=20000)
+        assert ei.value.code == 4 and ei.value.stats["overflow_units"] >= 1
+        engine.run(algorithm="nonblockinggroups", schedule=sched, spill_capacity=1 << 16,
+                   window=20000)
+        assert np.array_equal(engine.reports(), want)
 
 
 def test_reports_bit_exact_after_reload(engine):

@@ -1,0 +1,126 @@
+"""Runtime features on the GPU: graph replays, host work items, overflow
+fallback, the request server (alone == in the mix == oracle)."""
+import numpy as np
+import pytest
+
+import oracle
+from ngap_amd import NgapError, synth
+from ngap_amd.groups import make_group
+from ngap_amd.serve import Server, run_poisson
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _workload(seed=21, n=12):
+    rng = np.random.default_rng(seed)
+    groups = [synth.regex_group(rng, int(s), dotstar=(i % 4 == 0))
+              for i, s in enumerate(np.exp(rng.uniform(np.log(200), np.log(5000), n)))]
+    data = synth.zipf_bytes(rng, 1 << 17, 0.9)
+    return groups, data
+
+
+def test_graph_replays_match_eager(engine):
+    groups, data = _workload()
+    want, _ = oracle.run(groups, data, 1 << 13)
+    engine.load_groups(groups, 2)
+    engine.set_input(data, 1 << 13)
+    eager = dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
+                 unique=True, window=2048)
+    say("eager")
+    engine.run(**eager)
+    assert np.array_equal(engine.reports(), want)
+    opts = engine.options(use_graph=True, **eager)
+    for rep in range(4):
+        say("graph replay", rep)
+        engine.run(opts)
+        assert np.array_equal(engine.reports(), want)
+        # eager work between replays
+        engine.run(algorithm="nonblockinggroups", schedule="plain")
+        assert np.array_equal(engine.reports(), want)
+
+
+def test_host_work_items(engine):
+    groups, data = _workload(seed=22)
+    L = 1 << 13
+    S = len(data) // L
+    want, _ = oracle.run(groups, data, L)
+    engine.load_groups(groups, 3)
+    engine.set_input(data, L)
+    rng = np.random.default_rng(0)
+    tasks = []
+    for c in range(S):
+        for g in range(len(groups)):
+            cuts = sorted(set([0, L] + list(rng.integers(1, L, rng.integers(0, 5)))))
+            tasks += [(g, c, a, b) for a, b in zip(cuts, cuts[1:])]
+    order = rng.permutation(len(tasks))
+    tasks = [tasks[i] for i in order]
+    say("random tilings, random order")
+    st = engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=3,
+                    tasks=tasks)
+    assert st["num_tasks"] == len(tasks)
+    assert np.array_equal(engine.reports(), want)
+    assert engine.tasks().shape == (len(tasks), 4)
+    assert engine.task_items().shape == (len(tasks),)
+    # a gap or an overlap is rejected before anything runs
+    say("bad tilings")
+    with pytest.raises(NgapError):
+        engine.run(algorithm="nonblockingallgroups", schedule="batched", tasks=tasks[1:])
+    with pytest.raises(NgapError):
+        engine.run(algorithm="nonblockingallgroups", schedule="batched", tasks=tasks + tasks[:1])
+
+
+def test_overflow_fallback_restores_exact_reports(engine):
+    n = 1 + 4 + 16 + 64
+    edges = [(0, 1 + i) for i in range(4)] + [(1 + i, 5 + 4 * i + j) for i in range(4) for j in range(4)]
+    edges += [(5 + i, 21 + 4 * i + j) for i in range(16) for j in range(4)]
+    g = make_group(n, edges, [[ord("a")]] * n, all_input=[0], reporting=list(range(21, n)))
+    data = (b"a" * 3000 + b"b") * 4
+    want, _ = oracle.run([g], data, len(data) // 4, 4)
+    engine.load_groups([g], 1)
+    engine.set_input(data, len(data) // 4, 4)
+    say("fallback")
+    st = engine.run(algorithm="nonblockinggroups", schedule="batched", spill_capacity=64,
+                    window=1024, fallback=True)
+    assert st["fallback_tasks"] > 0
+    assert np.array_equal(engine.reports(), want)
+
+
+def test_server_alone_equals_mix_equals_oracle(engine):
+    groups = [synth.regex_group(np.random.default_rng(40 + i), 2000 + 1500 * i,
+                                dotstar=(i == 2)) for i in range(5)]
+    engine.load_groups(groups, 2)
+    rng = np.random.default_rng(7)
+    reqs = []
+    for i in range(24):
+        L = int(np.exp(rng.uniform(np.log(1024), np.log(1 << 17))))
+        reqs.append((int(rng.integers(0, len(groups))), synth.zipf_bytes(rng, L, 0.8)))
+    # oracle: each request as one chunk of its own group (group index 0 here,
+    # fixed up to the real group id)
+    expect = []
+    for g, d in reqs:
+        r, _ = oracle.run([groups[g]], d, len(d), 1)
+        expect.append(r | np.uint64(g << 22 << 32))
+    srv = Server(engine, max_inflight=3, algorithm="nonblockingallgroups", precompute_depth=2,
+                 unique=True)
+    say("server: requests alone")
+    alone = []
+    for g, d in reqs:
+        rid = srv.submit(g, d)
+        alone.append(srv.drain([rid])[rid][0])
+    say("server: requests in the mix")
+    ids = [srv.submit(g, d) for g, d in reqs]
+    mix = srv.drain(ids)
+    for i, (g, d) in enumerate(reqs):
+        assert np.array_equal(alone[i], expect[i]), i
+        assert np.array_equal(mix[ids[i]][0], expect[i]), i
+    say("server: poisson replay")
+    trace = [(0.002 * i, g, d) for i, (g, d) in enumerate(reqs)]
+    res = run_poisson(srv, trace)
+    assert res["requests"] == len(reqs) and res["p99_ms"] >= res["p50_ms"] > 0
+    for i in range(len(reqs)):
+        assert np.array_equal(res["reports"][i], expect[i])
+    srv.close()
