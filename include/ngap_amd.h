@@ -165,6 +165,30 @@ int ngap_server_take(ngap_server *srv, uint64_t req_id, int32_t *state, uint64_t
                      uint64_t cap, uint64_t *n_out, double *latency_ms);
 int ngap_server_pending(ngap_server *srv, uint64_t *queued, uint64_t *inflight);
 
+/* ---- learned schedule: frontier-size predictor (no reference counterpart) --
+ * A 264 -> 64 -> 64 -> 1 ReLU MLP in FP32 predicting log(1 + items) of a
+ * (group, 4 KiB window) work item from the window's normalised 256-bin byte
+ * histogram and 8 statistics of the group.  Parameters: W1[264][64], b1[64],
+ * W2[64][64], b2[64], W3[64], b3. */
+typedef struct ngap_mlp ngap_mlp;
+int ngap_mlp_create(int32_t device, uint64_t seed, ngap_mlp **out);
+int ngap_mlp_destroy(ngap_mlp *m);
+int32_t ngap_mlp_num_params(void);
+int ngap_mlp_get_params(ngap_mlp *m, float *out);
+int ngap_mlp_set_params(ngap_mlp *m, const float *in);
+/* y[n] for dense features x[n][264]. */
+int ngap_mlp_predict(ngap_mlp *m, const float *x, uint64_t n, float *y);
+/* y[n] for pairs[n] = (group, window): features hist[window][256] ++ stats[group][8]. */
+int ngap_mlp_predict_pairs(ngap_mlp *m, const float *hist, uint32_t num_windows,
+                           const float *stats, uint32_t num_groups, const uint32_t *pairs,
+                           uint64_t n, float *y);
+/* Gradient of the mean squared error over (x, t) into grad[num_params]. */
+int ngap_mlp_gradient(ngap_mlp *m, const float *x, const float *t, uint64_t n, float *grad,
+                      float *loss);
+/* Adam, mini-batches of `batch`, `epochs` passes; final epoch mean loss. */
+int ngap_mlp_train(ngap_mlp *m, const float *x, const float *t, uint64_t n, int32_t epochs,
+                   uint32_t batch, float lr, uint64_t seed, float *final_loss);
+
 /* ---- front-end: ANML -> groups in the reference state-id namespace ---- */
 typedef struct ngap_frontend ngap_frontend;
 

@@ -105,6 +105,18 @@ SIGNATURES = {
     "ngap_server_take": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32), C.c_void_p,
                                    C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "ngap_server_pending": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "ngap_mlp_create": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "ngap_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "ngap_mlp_num_params": (C.c_int32, []),
+    "ngap_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ngap_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ngap_mlp_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "ngap_mlp_predict_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                         C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "ngap_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                    C.POINTER(C.c_float)]),
+    "ngap_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32,
+                                 C.c_uint32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
     "ngap_frontend_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                      C.POINTER(C.c_void_p)]),
     "ngap_frontend_load_text": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

@@ -1,0 +1,127 @@
+"""Learned schedule for the batched engine.
+
+Traces: the batched engine run over a fixed tiling of 4 KiB windows reports,
+per (group, window) work item, the number of (state, offset) items it
+expanded.  Features: the window's normalised 256-bin byte histogram and 8
+statistics of the group (log states, log edges, log all-input states, log
+start-of-data states, log reporting states, mean out-degree, fraction of
+states whose class has >= 128 bytes, log self-loops).  Target: log(1 + items).
+
+The trained predictor (ngap_amd.mlp, HIP kernels) is used only to shape the
+schedule: consecutive windows of a (group, chunk) unit are merged into one
+work item until its predicted cost reaches a target, work items run
+longest-predicted-first, and the per-wave worklist spill capacity is sized
+from the predictions.  Matching is untouched; a work item whose worklist
+overflows is re-run by the engine with a larger worklist (fallback), and the
+report set is the same under every schedule.
+"""
+import numpy as np
+
+WINDOW = 4096
+SCAN_COST = 0.02  # cost of scanning one position, in units of one expanded item
+
+
+def windows_per_chunk(chunk, window=WINDOW):
+    return (chunk + window - 1) // window
+
+
+def window_hist(data, chunk, window=WINDOW):
+    """[num_windows, 256] normalised byte histograms; window index
+    w = c * windows_per_chunk + k covers chunk c, bytes [k*window, (k+1)*window)."""
+    data = np.asarray(data, np.uint8)
+    S = len(data) // chunk
+    K = windows_per_chunk(chunk, window)
+    out = np.zeros((S * K, 256), np.float32)
+    for c in range(S):
+        base = c * chunk
+        for k in range(K):
+            a, b = base + k * window, base + min((k + 1) * window, chunk)
+            h = np.bincount(data[a:b], minlength=256).astype(np.float32)
+            out[c * K + k] = h / max(1, b - a)
+    return out
+
+
+def group_stats(groups):
+    """[G, 8] float32 automaton statistics."""
+    out = np.zeros((len(groups), 8), np.float32)
+    for i, g in enumerate(groups):
+        n = max(1, g.num_states)
+        deg = np.diff(g.row)
+        src = np.repeat(np.arange(g.num_states), deg)
+        selfloops = int(np.sum(src == g.col))
+        width = np.unpackbits(np.ascontiguousarray(g.sets).view(np.uint8)
+                              .reshape(g.num_states, 32), axis=1).sum(1)
+        out[i] = [np.log1p(g.num_states), np.log1p(g.num_edges), np.log1p(g.aas.size),
+                  np.log1p(g.sod.size), np.log1p(int(np.sum(g.attrs & 2) // 2)),
+                  g.num_edges / n, float(np.mean(width >= 128)) if g.num_states else 0.0,
+                  np.log1p(selfloops)]
+    return out
+
+
+def fixed_tiling(num_groups, num_seg, chunk, window=WINDOW):
+    """Every (group, chunk, window) as one work item: uint32 [n, 4]."""
+    K = windows_per_chunk(chunk, window)
+    c, g, k = np.meshgrid(np.arange(num_seg), np.arange(num_groups), np.arange(K), indexing="ij")
+    begin = k * window
+    end = np.minimum(begin + window, chunk)
+    return np.stack([g.ravel(), c.ravel(), begin.ravel(), end.ravel()], 1).astype(np.uint32)
+
+
+def pairs_of(tasks, chunk, window=WINDOW):
+    """(group, global window index) of fixed-tiling work items."""
+    K = windows_per_chunk(chunk, window)
+    t = np.asarray(tasks, np.int64)
+    return np.stack([t[:, 0], t[:, 1] * K + t[:, 2] // window], 1).astype(np.uint32)
+
+
+def collect_traces(engine, data, chunk, num_groups, depth=2, window=WINDOW):
+    """Runs the fixed tiling once (inputs and groups already loaded) and
+    returns (pairs [n, 2], log1p(items) [n])."""
+    S = len(data) // chunk
+    tasks = fixed_tiling(num_groups, S, chunk, window)
+    engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=depth,
+               unique=True, tasks=tasks, fallback=True)
+    items = engine.task_items().astype(np.float64)
+    done = engine.tasks()
+    return pairs_of(done, chunk, window), np.log1p(items).astype(np.float32)
+
+
+def features(hist, stats, pairs):
+    pairs = np.asarray(pairs, np.int64)
+    return np.concatenate([hist[pairs[:, 1]], stats[pairs[:, 0]]], 1).astype(np.float32)
+
+
+def schedule_from_costs(cost, num_groups, num_seg, chunk, waves, window=WINDOW,
+                        per_wave=4):
+    """Merges consecutive windows of each (group, chunk) while the summed cost
+    stays under total / (waves * per_wave); returns work items ordered by
+    decreasing cost and their costs.  cost: [num_seg * K * ...] indexed by
+    (group, global window) as cost[g, w]."""
+    K = windows_per_chunk(chunk, window)
+    cost = np.asarray(cost, np.float64).reshape(num_groups, num_seg * K)
+    target = max(cost.sum() / max(1, waves * per_wave), 1e-9)
+    tasks, tcost = [], []
+    for g in range(num_groups):
+        for c in range(num_seg):
+            row = cost[g, c * K:(c + 1) * K]
+            a, acc = 0, 0.0
+            for k in range(K):
+                acc += row[k]
+                if acc >= target or k == K - 1:
+                    tasks.append((g, c, a * window, min((k + 1) * window, chunk)))
+                    tcost.append(acc)
+                    a, acc = k + 1, 0.0
+    order = np.argsort(-np.asarray(tcost), kind="stable")
+    return np.asarray(tasks, np.uint32)[order], np.asarray(tcost)[order]
+
+
+def predicted_cost(pred_log_items, window=WINDOW):
+    return np.expm1(np.maximum(pred_log_items, 0.0)) + SCAN_COST * window
+
+
+def spill_capacity(pred_log_items, tasks_cost, floor=2048):
+    """Worklist spill items per wave from the predictions: a quarter of the
+    largest predicted work item, rounded up to a power of two."""
+    peak = float(np.max(tasks_cost)) if len(tasks_cost) else 0.0
+    cap = max(floor, int(peak / 4))
+    return int(min(1 << 20, 1 << int(np.ceil(np.log2(cap)))))
