@@ -492,6 +492,120 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
 
+// blockinggroups (the reference's BAP baseline, kernel_ngap_O0.cu:7-126): one
+// workgroup per (group, chunk) unit advances the set of states matched at
+// position p to those matched at p + 1 in lock step, with a block barrier per
+// input byte.  The frontier lives in a per-unit HBM buffer (two halves of
+// P.spill_cap states); appends are compacted per wave.
+NGAP_DEV void block_append(const KParams &P, WaveState &W, uint32_t *count, uint32_t *buf,
+                           bool pred, uint32_t u) {
+  const uint64_t m = wballot(pred);
+  if (m == 0) return;
+  uint32_t base = 0;
+  if (W.lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, 0, 64);
+  const uint32_t slot = base + mask_rank(m);
+  if (pred) {
+    if (slot < P.spill_cap) buf[slot] = u;
+    else W.overflow = true;
+  }
+}
+
+__global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t unit = blockIdx.x;
+  const uint32_t g = unit % P.num_groups, c = unit / P.num_groups;
+  uint32_t *count = reinterpret_cast<uint32_t *>(lds);  // [0] current, [1] next
+  const uint32_t wbytes = wave_lds_bytes(P.ring_cap, P.rbuf_cap, P.dc_cap);
+  WaveState W;
+  setup_wave(P, W, lds + 16 + wave * wbytes, nullptr);
+  begin_unit(P, W, g, c * P.chunk_len, (c + 1) * P.chunk_len);
+  uint32_t *front = reinterpret_cast<uint32_t *>(P.spill) + (size_t)unit * 2 * P.spill_cap;
+  const GroupMeta gm = P.img.meta[g];
+  const bool seed = c == 0 || P.is_dup;
+  if (threadIdx.x == 0) count[0] = count[1] = 0;
+  __syncthreads();
+  // position cstart: matched start-of-data and all-input states
+  {
+    uint32_t *cur = front;
+    const uint32_t b = P.input[W.cstart];
+    for (uint32_t i = gm.sod_begin + wave * kWave; seed && i < gm.sod_end; i += kBlockThreads) {
+      const uint32_t idx = i + W.lane;
+      const bool ok = idx < gm.sod_end;
+      const int32_t e = ok ? P.img.sod[idx] : 0;
+      const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+      const bool hit = ok && class_has(P, W, u, b);
+      emit(P, W, hit && (e & kFlagReport), ((uint64_t)W.cstart << 32) | u | W.gtag);
+      block_append(P, W, &count[0], cur, hit && !(e & kFlagDeadEnd), u);
+    }
+    const uint32_t li = g * 256u + b;
+    const uint32_t lb = P.img.beg[0][li], le = P.img.beg[0][li + 1];
+    for (uint32_t i = lb + wave * kWave; i < le; i += kBlockThreads) {
+      const uint32_t idx = i + W.lane;
+      const bool ok = idx < le;
+      const int32_t e = ok ? P.img.list[0][idx] : 0;
+      const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+      emit(P, W, ok && (e & kFlagReport), ((uint64_t)W.cstart << 32) | u | W.gtag);
+      block_append(P, W, &count[0], cur, ok && !(e & kFlagDeadEnd), u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t p = W.cstart; p + 1 < W.cend; p++) {
+    const uint32_t sel = (p - W.cstart) & 1u;
+    uint32_t *cur = front + sel * P.spill_cap;
+    uint32_t *nxt = front + (sel ^ 1u) * P.spill_cap;
+    uint32_t *ncount = &count[sel ^ 1u];
+    const uint32_t n = min(count[sel], P.spill_cap);
+    const uint32_t p1 = p + 1;
+    const uint32_t b = P.input[p1];
+    // successors of the matched states, tested on input[p + 1]
+    for (uint32_t i = wave * kWave; i < n; i += kBlockThreads) {
+      const uint32_t idx = i + W.lane;
+      bool more = idx < n;
+      int4 rec = make_int4(-1, -1, -1, -1);
+      if (more) rec = reinterpret_cast<const int4 *>(P.img.succ4)[W.rec_base + cur[idx]];
+      while (wballot(more)) {
+        int32_t link = -1;
+        const int32_t es[4] = {rec.x, rec.y, rec.z, rec.w};
+#pragma unroll
+        for (int s2 = 0; s2 < 4; s2++) {
+          const int32_t e = es[s2];
+          if (more && e <= -2) link = -(e + 2);
+          const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+          bool hit = more && e >= 0 && class_has(P, W, u, b);
+          emit(P, W, hit && (e & kFlagReport), ((uint64_t)p1 << 32) | u | W.gtag);
+          bool keep = hit && !(e & kFlagDeadEnd);
+          if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, p1));
+          block_append(P, W, ncount, nxt, keep, u);
+        }
+        more = more && link >= 0;
+        if (more) rec = reinterpret_cast<const int4 *>(P.img.succ4)[link];
+      }
+    }
+    // all-input states matched at p + 1
+    const uint32_t li = g * 256u + b;
+    const uint32_t lb = P.img.beg[0][li], le = P.img.beg[0][li + 1];
+    for (uint32_t i = lb + wave * kWave; i < le; i += kBlockThreads) {
+      const uint32_t idx = i + W.lane;
+      const bool ok = idx < le;
+      const int32_t e = ok ? P.img.list[0][idx] : 0;
+      const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
+      emit(P, W, ok && (e & kFlagReport), ((uint64_t)p1 << 32) | u | W.gtag);
+      block_append(P, W, ncount, nxt, ok && !(e & kFlagDeadEnd), u);
+    }
+    if (wave == 0) W.items += n;
+    __syncthreads();
+    if (threadIdx.x == 0) count[sel] = 0;
+    __syncthreads();
+  }
+  flush_reports(P, W);
+  if (W.lane == 0) {
+    if (W.overflow) atomicOr(&P.overflow[unit], 1u);
+    atomicAdd(P.items, W.items);
+  }
+}
+
 // Clears the per-run counters with a kernel (graph-capture friendly).  A
 // re-run of overflowed work items keeps the report count (keep_reports).
 __global__ void k_reset(unsigned long long *report_count, unsigned long long *items,
@@ -522,6 +636,11 @@ hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_
   return hipGetLastError();
 }
 
+hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds) {
+  hipLaunchKernelGGL(k_blocking, dim3(units), dim3(kBlockThreads), lds, s, P);
+  return hipGetLastError();
+}
+
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds) {
   hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);
   return hipGetLastError();
@@ -535,6 +654,9 @@ hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds) {
 hipError_t set_lds_limit(uint32_t lds) {
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_plain),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_blocking),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),
                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);

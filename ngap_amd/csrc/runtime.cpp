@@ -25,6 +25,7 @@ namespace ngapamd {
 hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
                         uint32_t *head, uint32_t *ovf, uint32_t n, int keep_reports);
 hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
+hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
 hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds);
 hipError_t set_lds_limit(uint32_t lds);
@@ -93,8 +94,8 @@ bool map_algorithm(int algorithm, int depth_req, int loaded_depth, Mapping &m,
       m = {depth_req < 1 ? 1 : depth_req, -1};
       break;
     case NGAP_ALGO_BLOCKING:
-      err = "blockinggroups is not supported by this engine";
-      return false;
+      m = {1, 0};
+      break;
     default:
       err = "unknown algorithm";
       return false;
@@ -362,25 +363,32 @@ int host_tasks(const ngap_engine *eng, const ngap_run_opts &o, std::vector<Task>
   return NGAP_OK;
 }
 
+// One workgroup per (group, chunk) unit: the non-blocking engine (k_plain) or
+// the lock-step baseline (k_blocking, blockinggroups).
 int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
               ngap_run_stats *st) {
   const uint32_t G = (uint32_t)eng->img.num_groups, S = eng->num_seg;
   const uint32_t units = G * S;
+  const bool blocking = o.algorithm == NGAP_ALGO_BLOCKING;
   uint32_t spill = o.spill_capacity;
   if (!spill) {
-    // default budget: 8 GiB of spill for the whole grid, at most 64K items
+    // default budget: 8 GiB of spill (or frontier) for the whole grid, at
+    // most 64K entries per wave (per frontier half)
     uint64_t per = (8ull << 30) / 8 / ((uint64_t)units * P.waves_per_block);
     spill = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(1024, per));
   }
   P.spill_cap = spill;
-  HIP_TRY(ctx.spill.reserve((size_t)units * P.waves_per_block * spill * 8));
+  // k_plain: per-wave spill of u64 items; k_blocking: two u32 frontiers per unit
+  HIP_TRY(ctx.spill.reserve(blocking ? (size_t)units * 2 * spill * 4
+                                     : (size_t)units * P.waves_per_block * spill * 8));
   P.spill = ctx.spill.as<uint64_t>();
   HIP_TRY(ctx.overflow.reserve((size_t)units * 4));
   P.overflow = ctx.overflow.as<uint32_t>();
   attach_ctx(P, ctx);
   HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
   HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, units, 0));
-  HIP_TRY(launch_plain(ctx.stream, P, units, lds_bytes()));
+  if (blocking) HIP_TRY(launch_blocking(ctx.stream, P, units, lds_bytes()));
+  else HIP_TRY(launch_plain(ctx.stream, P, units, lds_bytes()));
   HIP_TRY(hipEventRecord(ctx.ev1, ctx.stream));
   HIP_TRY(hipEventSynchronize(ctx.ev1));
   float ms = 0;
@@ -545,7 +553,8 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   P.chunk_len = (uint32_t)eng->chunk_len;
   P.num_seg = eng->num_seg;
   if (st) st->num_units = (uint32_t)units;
-  if (o.schedule == NGAP_SCHED_PLAIN) return run_plain(eng, ctx, P, o, st);
+  if (o.schedule == NGAP_SCHED_PLAIN || o.algorithm == NGAP_ALGO_BLOCKING)
+    return run_plain(eng, ctx, P, o, st);
   std::vector<Task> tasks;
   if (o.tasks) {
     int rc = host_tasks(eng, o, tasks);

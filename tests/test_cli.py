@@ -46,7 +46,7 @@ def test_unknown_algorithm_message():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("algo", ["NAPgroups", "nonblockinggroups", "nonblockingr1groups",
+@pytest.mark.parametrize("algo", ["blockinggroups", "NAPgroups", "nonblockinggroups", "nonblockingr1groups",
                                   "nonblockingr2groups", "nonblockingpcgroups",
                                   "nonblockingallgroups"])
 @pytest.mark.parametrize("schedule", ["plain", "batched"])

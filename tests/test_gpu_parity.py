@@ -19,7 +19,7 @@ from tests import nfa_helpers as H
 
 pytestmark = pytest.mark.gpu
 
-ALGOS = ["NAPgroups", "nonblockinggroups", "nonblockingr1groups", "nonblockingr2groups",
+ALGOS = ["blockinggroups", "NAPgroups", "nonblockinggroups", "nonblockingr1groups", "nonblockingr2groups",
          "nonblockingpcgroups", "nonblockingallgroups"]
 
 
