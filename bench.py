@@ -109,6 +109,127 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def learned_comparison(depth, steps, workload_name):
+    """batch1024 / learned workloads: plain (one workgroup per unit) vs the
+    batched engine with its static schedule vs the batched engine with the
+    MLP-shaped schedule; MLP trained on traces of an input with a different
+    seed.  Returns a dict (MB/s, speedups, MAE, overflow rate, bit-exactness)."""
+    from ngap_amd import Engine, learned, synth
+    from ngap_amd.mlp import MLP
+    t0 = time.time()
+    rng = np.random.default_rng(1002 if workload_name == "batch1024" else 1003)
+    groups = synth.batch_groups(rng, 1024)
+    chunk = 64 << 10
+    n = 8 << 20
+    if workload_name == "batch1024":
+        test = synth.zipf_bytes(np.random.default_rng(2002), n, 1.1)
+        train = synth.zipf_bytes(np.random.default_rng(3002), n, 1.1)
+    else:
+        test = synth.drifting_input(np.random.default_rng(2003), n, chunk, 1e-4, groups)
+        train = synth.drifting_input(np.random.default_rng(3003), n, chunk, 1e-4, groups)
+    log(f"{workload_name}: {sum(g.num_states for g in groups)} states, generated in "
+        f"{time.time() - t0:.1f} s")
+    eng = Engine(0)
+    eng.load_groups(groups, max(depth, 1))
+    stats = learned.group_stats(groups)
+    # traces of the training input -> at most 10^6 samples
+    eng.set_input(train, chunk)
+    t1 = time.time()
+    pairs, y = learned.collect_traces(eng, train, chunk, len(groups), depth)
+    x = learned.features(learned.window_hist(train, chunk), stats, pairs)
+    if x.shape[0] > 1000000:
+        keep = np.random.default_rng(0).choice(x.shape[0], 1000000, replace=False)
+        x, y = x[keep], y[keep]
+    m = MLP(0, seed=7)
+    loss = m.train(x, y, epochs=4, batch=4096, lr=2e-3)
+    log(f"trained on {x.shape[0]} traces in {time.time() - t1:.1f} s, loss {loss:.4f}")
+    # test input
+    eng.set_input(test, chunk)
+    total = len(test)
+    common = dict(algorithm="nonblockingallgroups", precompute_depth=depth, unique=True)
+    out = {"workload": workload_name, "groups": len(groups),
+           "states": int(sum(g.num_states for g in groups)), "input_bytes": total,
+           "chunk": chunk, "train_traces": int(x.shape[0]), "train_loss": loss}
+    res = {}
+    wall, kern, st = timed(eng, eng.options(schedule="plain", **common), steps, 1)
+    res["plain"] = (wall, eng.reports())
+    wall, kern, st = timed(eng, eng.options(schedule="batched", **common), steps, 1)
+    res["static"] = (wall, eng.reports())
+    t2 = time.time()
+    S = total // chunk
+    K = learned.windows_per_chunk(chunk)
+    all_pairs = np.stack(np.meshgrid(np.arange(len(groups)), np.arange(S * K), indexing="ij"),
+                         -1).reshape(-1, 2).astype(np.uint32)
+    hist = learned.window_hist(test, chunk)
+    pred = m.predict_pairs(hist, stats, all_pairs)
+    cost = learned.predicted_cost(pred).reshape(len(groups), S * K)
+    blocks_waves = 256 * 4 * 4
+    tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, blocks_waves)
+    spill = learned.spill_capacity(pred, tcost)
+    sched_s = time.time() - t2
+    opts = eng.options(schedule="batched", tasks=tasks, spill_capacity=spill, fallback=True,
+                       use_graph=True, **common)
+    wall, kern, st = timed(eng, opts, steps, 1)
+    res["learned"] = (wall, eng.reports())
+    out["learned_fallback_tasks"] = int(st["fallback_tasks"])
+    out["learned_overflow_rate"] = st["fallback_tasks"] / max(1, st["num_tasks"])
+    out["learned_tasks"] = int(st["num_tasks"])
+    out["learned_spill_capacity"] = spill
+    out["schedule_host_s"] = sched_s
+    # prediction quality on the test input
+    tp, ty = learned.collect_traces(eng, test, chunk, len(groups), depth)
+    tpred = m.predict_pairs(hist, stats, tp)
+    out["mae_log_items"] = float(np.mean(np.abs(tpred - ty)))
+    out["mae_constant_predictor"] = float(np.mean(np.abs(np.mean(y) - ty)))
+    for k, (wall, _) in res.items():
+        out[f"{k}_mbps"] = total / (wall * 1e-3) / 1e6
+    out["speedup_learned_vs_plain"] = res["plain"][0] / res["learned"][0]
+    out["speedup_static_vs_plain"] = res["plain"][0] / res["static"][0]
+    out["bit_exact"] = bool(np.array_equal(res["plain"][1], res["static"][1]) and
+                            np.array_equal(res["plain"][1], res["learned"][1]))
+    out["reports"] = int(res["plain"][1].size)
+    log(f"{workload_name}: {out}")
+    m.close()
+    eng.close()
+    return out
+
+
+def serving_bench(rates, n_requests=128):
+    """Serving workload: 64 resident automata (1k..200k states), Poisson
+    requests of 1 KiB..4 MiB Zipf(1.1) bytes; p50/p99 latency and aggregate
+    MB/s per arrival rate; every request's reports checked against the same
+    request processed alone."""
+    from ngap_amd import Engine, synth
+    from ngap_amd.serve import Server, run_poisson
+    t0 = time.time()
+    groups = synth.serving_groups()
+    eng = Engine(0)
+    eng.load_groups(groups, 2)
+    log(f"serving: {sum(g.num_states for g in groups)} states loaded in {time.time() - t0:.1f} s")
+    srv = Server(eng, budget_bytes=200 << 30, max_inflight=4, algorithm="nonblockingallgroups",
+                 precompute_depth=2, unique=True)
+    base = synth.serving_requests(seed=2004, n=n_requests)
+    alone = []
+    for _, g, d in base:
+        rid = srv.submit(g, d)
+        alone.append(srv.drain([rid])[rid][0])
+    out = {"groups": len(groups), "requests": n_requests,
+           "input_bytes": int(sum(len(d) for _, _, d in base)), "rates": []}
+    for rate in rates:
+        rng = np.random.default_rng(int(rate))
+        t = np.cumsum(rng.exponential(1.0 / rate, len(base)))
+        trace = [(float(t[i]), g, d) for i, (_, g, d) in enumerate(base)]
+        res = run_poisson(srv, trace)
+        exact = all(np.array_equal(res["reports"][i], alone[i]) for i in range(len(base)))
+        row = {k: res[k] for k in ("p50_ms", "p99_ms", "mean_ms", "mbps", "makespan_s")}
+        row.update(rate_per_s=rate, bit_exact_vs_alone=exact)
+        out["rates"].append(row)
+        log(f"serving rate {rate}/s: {row}")
+    srv.close()
+    eng.close()
+    return out
+
+
 def timed(eng, opts, steps, warmup):
     for _ in range(warmup):
         eng.run(opts)
@@ -134,6 +255,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["plain", "batched"], default=None)
+    ap.add_argument("--learned", default="learned", choices=["none", "batch1024", "learned"],
+                    help="also run the plain / static / learned comparison on this workload")
+    ap.add_argument("--serving", default="100,1000,10000",
+                    help="arrival rates (per second) for the serving run; 'none' skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     ap.add_argument("--oracle-check", action="store_true",
@@ -224,9 +349,14 @@ def main():
         "bit_exact_batched_vs_plain": bit_exact, "bit_exact_vs_oracle_sampled": oracle_check,
         "roofline": roofline,
     }
+    eng.close()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(groups, data, chunk)
         log(f"cpu baseline: {out['cpu_baseline']}")
+    if a.learned != "none":
+        out["learned"] = learned_comparison(max(1, a.depth), a.steps, a.learned)
+    if a.serving != "none":
+        out["serving"] = serving_bench([float(r) for r in a.serving.split(",")])
     print(json.dumps(out), flush=True)
 
 
