@@ -56,9 +56,12 @@ extern "C" {
 typedef struct ngap_engine ngap_engine;
 
 /* One batched work item: roots at positions [begin, end) of chunk `chunk`
- * (offsets relative to the chunk start) for group `group`; its items run to
- * the end of the chunk.  The work items of one (group, chunk) unit must tile
- * [0, chunk_len) exactly. */
+ * (offsets relative to the chunk start) for group `group`, or for every group
+ * when group is NGAP_ALL_GROUPS (precompute depth >= 2); its items run to the
+ * end of the chunk.  The work items of one (group, chunk) unit must tile
+ * [0, chunk_len) exactly; a chunk with an all-group work item takes only
+ * all-group work items. */
+#define NGAP_ALL_GROUPS 0xFFFFFFFFu
 typedef struct {
   uint32_t group, chunk, begin, end;
 } ngap_task;
@@ -99,6 +102,10 @@ typedef struct {
   uint32_t num_tasks;
   int32_t fallback;             /* re-run overflowing work items with a larger
                                    worklist instead of failing */
+  int32_t cross_groups;         /* batched static schedule: 1 = work items span
+                                   all groups when there are several and the
+                                   precompute depth is >= 2, 2 = also for one
+                                   group, 0 = one group per work item */
 } ngap_run_opts;
 
 typedef struct {

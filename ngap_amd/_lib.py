@@ -68,6 +68,7 @@ class RunOpts(C.Structure):
         ("tasks", C.c_void_p),
         ("num_tasks", C.c_uint32),
         ("fallback", C.c_int32),
+        ("cross_groups", C.c_int32),
     ]
 
 

@@ -21,10 +21,11 @@ struct DevImage {
   const uint32_t *beg[kMaxMemoDepth];
   const uint32_t *nrep[kMaxMemoDepth];
   const int32_t *list[kMaxMemoDepth];
-  const uint32_t *key[kMaxMemoDepth];
-  const uint32_t *dir[kMaxMemoDepth];
+  const uint32_t *sel[kMaxMemoDepth];  // rank tables of depths 2 and 3
   const uint32_t *mask1;     // 8 words per group
-  const uint32_t *mask2;     // 2048 words per group (depth >= 2 built)
+  // cross-group root index (layout.h)
+  const uint32_t *xr1, *xe1, *xmask1, *xr2, *xe2, *xsod;
+  uint32_t n_xsod;
 };
 
 // One batched work item: roots at positions [begin, end) of the chunk
@@ -34,6 +35,9 @@ struct Task {
   uint32_t group, cstart, cend, begin, end, flags;
 };
 constexpr uint32_t kTaskSeedStart = 1;
+// Task.group of a work item that scans its window for every group at once
+// (roots from the cross-group index, memo depth >= 2).
+constexpr uint32_t kAllGroups = 0xFFFFFFFFu;
 
 struct KParams {
   DevImage img;
@@ -68,7 +72,7 @@ constexpr uint32_t kBlockThreads = 256;
 
 // LDS bytes per wave for the given capacities
 NGAP_HD inline uint32_t wave_lds_bytes(uint32_t ring, uint32_t rbuf, uint32_t dc) {
-  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64 + 8) * 4 + 4;
+  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64 + 64 + 8) * 4 + 4;
 }
 
 }  // namespace ngapamd

@@ -65,7 +65,8 @@ struct WaveState {
   uint32_t *sh_pref;  // 65
   uint32_t *sh_beg;   // 64
   uint32_t *sh_pos;   // 64
-  uint32_t *mask1;    // 8: bytes that start a root in the current group
+  uint32_t *sh_tag;   // 64
+  uint32_t *mask1;    // 8: bytes that start a root in the current group(s)
   uint64_t *spill;
   uint32_t head, tail, rn, spill_n;
   uint32_t lane;
@@ -73,14 +74,16 @@ struct WaveState {
   unsigned long long items;
   // current (group, chunk)
   uint32_t g, gtag, rec_base, cls_base, cstart, cend;
-  uint32_t key_base2, key_base3;
 };
 
-NGAP_DEV bool class_has(const KParams &P, const WaveState &W, uint32_t u, uint32_t b) {
-  return (P.img.cls[(size_t)(W.cls_base + u) * 8 + (b >> 5)] >> (b & 31)) & 1u;
+NGAP_DEV bool class_has(const KParams &P, uint32_t cls_base, uint32_t u, uint32_t b) {
+  return (P.img.cls[(size_t)(cls_base + u) * 8 + (b >> 5)] >> (b & 31)) & 1u;
 }
 
-NGAP_DEV uint64_t item_of(uint32_t u, uint32_t pos) { return ((uint64_t)pos << 32) | u; }
+// An item (and a report) is (offset << 32) | (group << 22) | state.
+NGAP_DEV uint64_t item_of(uint32_t tagged, uint32_t pos) {
+  return ((uint64_t)pos << 32) | tagged;
+}
 
 NGAP_DEV void push(const KParams &P, WaveState &W, bool pred, uint64_t item) {
   uint64_t m = wballot(pred);
@@ -151,20 +154,21 @@ NGAP_DEV void refill_from_spill(const KParams &P, WaveState &W) {
 // list is report-only (bit 31 of beg), the state has no successors, or the
 // position is the chunk's last.
 NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg,
-                      uint32_t pos, const int32_t *list) {
+                      uint32_t pos, uint32_t gtag, const int32_t *list) {
   uint32_t incl = wave_incl_scan(cnt, W.lane);
   uint32_t total = __shfl(incl, 63, 64);
   if (total == 0) return;
   W.sh_pref[W.lane] = incl - cnt;
   W.sh_beg[W.lane] = beg;
   W.sh_pos[W.lane] = pos;
+  W.sh_tag[W.lane] = gtag;
   if (W.lane == 63) W.sh_pref[64] = total;
   wsync();
   for (uint32_t base = 0; base < total; base += kWave) {
     uint32_t j = base + W.lane;
     bool ok = j < total;
     int32_t e = 0;
-    uint32_t ppos = 0;
+    uint32_t ppos = 0, tg = 0;
     bool ronly = true;
     if (ok) {
       int lo = 0, hi = 64;  // largest o with pref[o] <= j (pref[64] = total > j)
@@ -177,9 +181,10 @@ NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg
       ronly = bg >> 31;
       e = list[(bg & 0x7fffffffu) + (j - W.sh_pref[lo])];
       ppos = W.sh_pos[lo];
+      tg = W.sh_tag[lo];
     }
-    const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-    emit(P, W, ok && (e & kFlagReport), ((uint64_t)ppos << 32) | u | W.gtag);
+    const uint32_t u = ((uint32_t)e & (uint32_t)kIdMask) | tg;
+    emit(P, W, ok && (e & kFlagReport), item_of(u, ppos));
     bool keep = ok && !ronly && !(e & kFlagDeadEnd) && (ppos + 1 < W.cend);
     if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, ppos));
     push(P, W, keep, item_of(u, ppos));
@@ -187,22 +192,12 @@ NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg
   wsync();
 }
 
-// Index of the depth-d (2 or 3) list of a d-gram key, -1 if it has none:
-// the first byte selects a bucket of sorted keys, a binary search finds it.
-NGAP_DEV int32_t memo_lookup(const KParams &P, const WaveState &W, int d, uint32_t key) {
-  const uint32_t first = key >> (8 * (d - 1));
-  const uint32_t *dir = P.img.dir[d - 1] + (size_t)W.g * 257;
-  const uint32_t kb = d == 2 ? W.key_base2 : W.key_base3;
-  int lo = (int)(kb + dir[first]), hi = (int)(kb + dir[first + 1]) - 1;
-  const uint32_t *keys = P.img.key[d - 1];
-  while (lo <= hi) {
-    int mid = (lo + hi) >> 1;
-    uint32_t k = keys[mid];
-    if (k == key) return mid;
-    if (k < key) lo = mid + 1;
-    else hi = mid - 1;
-  }
-  return -1;
+// Rank-table lookup: pair `at` covers 32 consecutive keys; returns the
+// key's list index, or -1 when bit `bit` of the pair's word is clear.
+NGAP_DEV int32_t memo_rank(const uint32_t *sel, size_t at, uint32_t bit) {
+  const uint2 w = reinterpret_cast<const uint2 *>(sel)[at];
+  if (!((w.x >> bit) & 1u)) return -1;
+  return (int32_t)(w.y + __popc(w.x & ((1u << bit) - 1u)));
 }
 
 // Roots for positions [p0, pend): one lane per position.  A position whose
@@ -228,44 +223,138 @@ NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) 
       cnt = k == 1 ? P.img.beg[0][li + 1] - beg : P.img.nrep[0][li];
       if (k != 1) beg |= kReportOnly;
     }
-    flatten(P, W, cnt, beg, p, P.img.list[0]);
+    flatten(P, W, cnt, beg, p, W.gtag, P.img.list[0]);
   }
   if (P.memo_depth < 2) return;
-  // depth 2: the 2-gram filter, then the bucketed key index
+  // depth 2: the group's 2-gram rank table
   bool has2 = has && k >= 2;
-  uint32_t key = 0;
+  int32_t li2 = -1;
   if (has2) {
-    key = (b0 << 8) | in[p + 1];
-    has2 = (P.img.mask2[(size_t)W.g * 2048 + (key >> 5)] >> (key & 31)) & 1u;
+    const uint32_t key = (b0 << 8) | in[p + 1];
+    li2 = memo_rank(P.img.sel[1], (size_t)W.g * 2048 + (key >> 5), key & 31);
+    has2 = li2 >= 0;
   }
   if (wballot(has2) == 0) return;
   {
     uint32_t beg = 0, cnt = 0;
     if (has2) {
-      const int32_t li = memo_lookup(P, W, 2, key);
-      if (li >= 0) {
-        beg = P.img.beg[1][li];
-        cnt = k == 2 ? P.img.beg[1][li + 1] - beg : P.img.nrep[1][li];
-        if (k != 2) beg |= kReportOnly;
-      } else {
-        has2 = false;
-      }
+      beg = P.img.beg[1][li2];
+      cnt = k == 2 ? P.img.beg[1][li2 + 1] - beg : P.img.nrep[1][li2];
+      if (k != 2) beg |= kReportOnly;
     }
-    flatten(P, W, cnt, beg, p + 1, P.img.list[1]);
+    flatten(P, W, cnt, beg, p + 1, W.gtag, P.img.list[1]);
   }
   if (P.memo_depth < 3) return;
+  // depth 3: the rank table below the first two bytes' depth-2 key
   bool has3 = has2 && k >= 3;
+  int32_t li3 = -1;
+  if (has3) {
+    const uint32_t b2 = in[p + 2];
+    li3 = memo_rank(P.img.sel[2], (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
+    has3 = li3 >= 0;
+  }
+  if (wballot(has3) == 0) return;
   {
     uint32_t beg = 0, cnt = 0;
     if (has3) {
-      const int32_t li = memo_lookup(P, W, 3, (key << 8) | in[p + 2]);
-      if (li >= 0) {
-        beg = P.img.beg[2][li];
-        cnt = P.img.beg[2][li + 1] - beg;
-      }
+      beg = P.img.beg[2][li3];
+      cnt = P.img.beg[2][li3 + 1] - beg;
     }
-    flatten(P, W, cnt, beg, p + 2, P.img.list[2]);
+    flatten(P, W, cnt, beg, p + 2, W.gtag, P.img.list[2]);
   }
+}
+
+// Distributes per-lane index ranges [e0, e0 + ecnt) over the wave, 64
+// entries per round, and calls f(ok, entry index, owner lane) each round (a
+// wave-uniform loop; the owner is found by binary search over the inclusive
+// prefix sums held in the lanes).
+template <class F>
+NGAP_DEV void for_entries(const WaveState &W, uint32_t e0, uint32_t ecnt, F &&f) {
+  const uint32_t incl = wave_incl_scan(ecnt, W.lane);
+  const uint32_t total = __shfl(incl, 63, 64);
+  for (uint32_t base = 0; base < total; base += kWave) {
+    const uint32_t j = base + W.lane;
+    const bool ok = j < total;
+    uint32_t lo = 0, hi = 63;  // smallest lane o with incl[o] > j
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (__shfl(incl, (int)mid, 64) > j) hi = mid;
+      else lo = mid + 1;
+    }
+    const uint32_t oi = __shfl(incl, (int)lo, 64);
+    const uint32_t oc = __shfl(ecnt, (int)lo, 64);
+    const uint32_t oe = __shfl(e0, (int)lo, 64);
+    f(ok, oe + j - (oi - oc), lo);
+  }
+}
+
+// Roots of every group for positions [p0, pend) (cross-group work items,
+// memo depth >= 2): one lane per position; the position's byte selects the
+// groups whose all-input states report on it, its 2-gram the depth-2 lists of
+// every group, each continued at depth 3 through that list's rank table.
+// The same lists as roots() visits group by group, so the same items.
+NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
+  const uint8_t *in = P.input;
+  const uint32_t p = p0 + W.lane;
+  const bool act = p < pend;
+  const uint32_t b0 = act ? in[p] : 0u;
+  const uint32_t left = act ? W.cend - p : 0u;
+  const uint32_t k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
+  const uint32_t kReportOnly = 1u << 31;
+  {  // depth 1: reporting entries only (positions that continue use depth >= 2)
+    uint32_t e0 = 0, ec = 0;
+    if (act && ((W.mask1[b0 >> 5] >> (b0 & 31)) & 1u)) {
+      e0 = P.img.xr1[b0];
+      ec = P.img.xr1[b0 + 1] - e0;
+    }
+    if (wballot(ec != 0)) {
+      for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
+        const uint32_t pos = __shfl(p, (int)o, 64);
+        uint32_t beg = 0, cnt = 0, tag = 0;
+        if (ok) {
+          const uint32_t li = P.img.xe1[idx];
+          beg = P.img.beg[0][li] | kReportOnly;
+          cnt = P.img.nrep[0][li];
+          tag = (li >> 8) << 22;
+        }
+        flatten(P, W, cnt, beg, pos, tag, P.img.list[0]);
+      });
+    }
+  }
+  uint32_t e0 = 0, ec = 0;
+  if (act && k >= 2) {
+    const uint32_t key = (b0 << 8) | in[p + 1];
+    e0 = P.img.xr2[key];
+    ec = P.img.xr2[key + 1] - e0;
+  }
+  if (wballot(ec != 0) == 0) return;
+  for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
+    const uint32_t pos = __shfl(p, (int)o, 64);
+    const uint32_t ko = __shfl(k, (int)o, 64);
+    uint32_t beg = 0, cnt = 0, tag = 0, li2 = 0;
+    if (ok) {
+      const uint2 e = reinterpret_cast<const uint2 *>(P.img.xe2)[idx];
+      li2 = e.x;
+      tag = e.y << 22;
+      beg = P.img.beg[1][li2];
+      cnt = ko == 2 ? P.img.beg[1][li2 + 1] - beg : P.img.nrep[1][li2];
+      if (ko != 2) beg |= kReportOnly;
+    }
+    flatten(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
+    if (P.memo_depth >= 3) {
+      uint32_t beg3 = 0, cnt3 = 0;
+      if (ok && ko >= 3) {
+        const uint32_t b2 = in[pos + 2];
+        const int32_t li3 = memo_rank(P.img.sel[2], (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
+        if (li3 >= 0) {
+          beg3 = P.img.beg[2][li3];
+          cnt3 = P.img.beg[2][li3 + 1] - beg3;
+        }
+      }
+      flatten(P, W, cnt3, beg3, pos + 2, tag, P.img.list[2]);
+    }
+  });
 }
 
 NGAP_DEV void seed_start_of_data(const KParams &P, WaveState &W, uint32_t sod_begin,
@@ -276,23 +365,53 @@ NGAP_DEV void seed_start_of_data(const KParams &P, WaveState &W, uint32_t sod_be
     bool ok = idx < sod_end;
     int32_t e = ok ? P.img.sod[idx] : 0;
     uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-    bool hit = ok && class_has(P, W, u, b);
-    emit(P, W, hit && (e & kFlagReport), ((uint64_t)W.cstart << 32) | u | W.gtag);
+    bool hit = ok && class_has(P, W.cls_base, u, b);
+    const uint64_t it = item_of(u | W.gtag, W.cstart);
+    emit(P, W, hit && (e & kFlagReport), it);
     bool keep = hit && !(e & kFlagDeadEnd) && (W.cstart + 1 < W.cend);
-    if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, W.cstart));
-    push(P, W, keep, item_of(u, W.cstart));
+    if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
+    push(P, W, keep, it);
+  }
+}
+
+// The same for every group (cross-group work items): (tagged id, group)
+// pairs, each group's class base from its metadata.
+NGAP_DEV void seed_start_all(const KParams &P, WaveState &W) {
+  const uint32_t b = P.input[W.cstart];
+  for (uint32_t i = 0; i < P.img.n_xsod; i += kWave) {
+    const uint32_t idx = i + W.lane;
+    const bool ok = idx < P.img.n_xsod;
+    int32_t e = 0;
+    uint32_t g = 0;
+    bool hit = false;
+    if (ok) {
+      e = (int32_t)P.img.xsod[2 * idx];
+      g = P.img.xsod[2 * idx + 1];
+      hit = class_has(P, P.img.meta[g].cls_base, (uint32_t)e & (uint32_t)kIdMask, b);
+    }
+    const uint64_t it = item_of(((uint32_t)e & (uint32_t)kIdMask) | (g << 22), W.cstart);
+    emit(P, W, hit && (e & kFlagReport), it);
+    bool keep = hit && !(e & kFlagDeadEnd) && (W.cstart + 1 < W.cend);
+    if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
+    push(P, W, keep, it);
   }
 }
 
 // One step of every valid lane's item: test its successors on the next byte.
+// Lane context of the item being expanded: group tag and the group's record
+// and class bases.
+struct LaneGroup {
+  uint32_t gtag, rec_base, cls_base;
+};
+
 NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
-                     uint32_t &p, int &depth) {
+                     uint32_t &p, int &depth, const LaneGroup &lg) {
   const uint32_t p1 = p + 1;
   uint32_t b = 0;
   int4 rec = make_int4(-1, -1, -1, -1);
   if (valid) {
     b = P.input[p1];
-    rec = reinterpret_cast<const int4 *>(P.img.succ4)[W.rec_base + v];
+    rec = reinterpret_cast<const int4 *>(P.img.succ4)[lg.rec_base + v];
   }
   W.items += __popcll(wballot(valid));
   bool has_next = false;
@@ -309,16 +428,17 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
       if (more && e <= -2) link = -(e + 2);
       const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
       bool hit = false;
-      if (edge) hit = class_has(P, W, u, b);
-      emit(P, W, hit && (e & kFlagReport), ((uint64_t)p1 << 32) | u | W.gtag);
+      if (edge) hit = class_has(P, lg.cls_base, u, b);
+      const uint64_t it = item_of(u | lg.gtag, p1);
+      emit(P, W, hit && (e & kFlagReport), it);
       bool keep = hit && !(e & kFlagDeadEnd) && (p1 + 1 < W.cend);
-      if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, p1));
+      if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
       const bool take = keep && !has_next && can_priv;
       if (take) {
         has_next = true;
         nv = u;
       }
-      push(P, W, keep && !take, item_of(u, p1));
+      push(P, W, keep && !take, it);
     }
     more = more && link >= 0;
     if (more) rec = reinterpret_cast<const int4 *>(P.img.succ4)[link];
@@ -339,12 +459,21 @@ NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t cs
   W.gtag = g << 22;
   W.rec_base = gm.rec_base;
   W.cls_base = gm.cls_base;
-  W.key_base2 = gm.memo_key_base[0];
-  W.key_base3 = gm.memo_key_base[1];
   W.cstart = cstart;
   W.cend = cend;
   for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
   if (W.lane < 8) W.mask1[W.lane] = P.img.mask1[g * 8 + W.lane];
+  wsync();
+}
+
+// A cross-group work item: every group's roots in one window of a chunk.
+NGAP_DEV void begin_cross(const KParams &P, WaveState &W, uint32_t cstart, uint32_t cend) {
+  W.g = kAllGroups;
+  W.gtag = 0;
+  W.cstart = cstart;
+  W.cend = cend;
+  for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
+  if (W.lane < 8) W.mask1[W.lane] = P.img.xmask1[W.lane];
   wsync();
 }
 
@@ -356,7 +485,8 @@ NGAP_DEV void setup_wave(const KParams &P, WaveState &W, unsigned char *base,
   W.sh_pref = reinterpret_cast<uint32_t *>(W.dc + P.dc_cap);
   W.sh_beg = W.sh_pref + 65;
   W.sh_pos = W.sh_beg + 64;
-  W.mask1 = W.sh_pos + 64;
+  W.sh_tag = W.sh_pos + 64;
+  W.mask1 = W.sh_tag + 64;
   W.spill = spill;
   W.head = W.tail = W.rn = W.spill_n = 0;
   W.lane = lane_id();
@@ -366,11 +496,12 @@ NGAP_DEV void setup_wave(const KParams &P, WaveState &W, unsigned char *base,
 
 // Runs items to completion, pulling root batches from `src` whenever the
 // ring runs low.  Every loop iteration is wave-uniform.
-template <class Src>
+template <bool Cross, class Src>
 NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
   bool valid = false;
   uint32_t v = 0, p = 0;
   int depth = 0;
+  LaneGroup lg{W.gtag, W.rec_base, W.cls_base};
   bool roots_left = true;
   for (;;) {
     uint32_t qn = W.tail - W.head;
@@ -380,8 +511,9 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
     }
     if (qn < kWave && W.spill_n == 0 && roots_left) {
       uint32_t a, b;
-      if (src.next(W, a, b)) roots(P, W, a, b);
-      else roots_left = false;
+      if (!src.next(W, a, b)) roots_left = false;
+      else if (Cross) roots_cross(P, W, a, b);
+      else roots(P, W, a, b);
       continue;
     }
     const uint64_t idle = wballot(!valid);
@@ -391,10 +523,17 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
     uint32_t r = mask_rank(idle);
     if (!valid && r < take) {
       uint64_t it = W.ring[(W.head + r) & (P.ring_cap - 1)];
-      v = (uint32_t)it;
+      v = (uint32_t)it & (uint32_t)kIdMask;
       p = (uint32_t)(it >> 32);
       valid = true;
       depth = 0;
+      if (Cross) {
+        const uint32_t g = ((uint32_t)it >> 22) & 1023u;
+        const uint2 base = *reinterpret_cast<const uint2 *>(&P.img.meta[g]);
+        lg.gtag = g << 22;
+        lg.rec_base = base.x;
+        lg.cls_base = base.y;
+      }
     }
     W.head += take;
     wsync();
@@ -402,7 +541,7 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
       if (W.tail == W.head && W.spill_n == 0 && !roots_left) break;
       continue;
     }
-    expand(P, W, valid, v, p, depth);
+    expand(P, W, valid, v, p, depth, lg);
   }
 }
 
@@ -452,7 +591,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_plain(KParams P) {
     seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
   }
   SharedCursor src{cursor, W.cend, P.root_batch};
-  drain(P, W, src);
+  drain<false>(P, W, src);
   flush_reports(P, W);
   if (W.lane == 0) {
     if (W.overflow) atomicOr(&P.overflow[unit], 1u);
@@ -474,15 +613,21 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
     if (t >= P.num_tasks) break;
     const uint32_t ti = P.task_order ? P.task_order[t] : t;
     const Task T = P.tasks[ti];
-    begin_unit(P, W, T.group, T.cstart, T.cend);
     W.overflow = false;
     const unsigned long long items0 = W.items;
-    if (T.flags & kTaskSeedStart) {
-      const GroupMeta gm = P.img.meta[T.group];
-      seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
-    }
     LocalCursor src{T.begin, T.end};
-    drain(P, W, src);
+    if (T.group == kAllGroups) {
+      begin_cross(P, W, T.cstart, T.cend);
+      if (T.flags & kTaskSeedStart) seed_start_all(P, W);
+      drain<true>(P, W, src);
+    } else {
+      begin_unit(P, W, T.group, T.cstart, T.cend);
+      if (T.flags & kTaskSeedStart) {
+        const GroupMeta gm = P.img.meta[T.group];
+        seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
+      }
+      drain<false>(P, W, src);
+    }
     if (W.lane == 0) {
       if (W.overflow) P.overflow[ti] = 1u;
       if (P.task_items) P.task_items[ti] = W.items - items0;
@@ -535,7 +680,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
       const bool ok = idx < gm.sod_end;
       const int32_t e = ok ? P.img.sod[idx] : 0;
       const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-      const bool hit = ok && class_has(P, W, u, b);
+      const bool hit = ok && class_has(P, W.cls_base, u, b);
       emit(P, W, hit && (e & kFlagReport), ((uint64_t)W.cstart << 32) | u | W.gtag);
       block_append(P, W, &count[0], cur, hit && !(e & kFlagDeadEnd), u);
     }
@@ -573,10 +718,10 @@ __global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
           const int32_t e = es[s2];
           if (more && e <= -2) link = -(e + 2);
           const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-          bool hit = more && e >= 0 && class_has(P, W, u, b);
-          emit(P, W, hit && (e & kFlagReport), ((uint64_t)p1 << 32) | u | W.gtag);
+          bool hit = more && e >= 0 && class_has(P, W.cls_base, u, b);
+          emit(P, W, hit && (e & kFlagReport), item_of(u | W.gtag, p1));
           bool keep = hit && !(e & kFlagDeadEnd);
-          if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, p1));
+          if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u | W.gtag, p1));
           block_append(P, W, ncount, nxt, keep, u);
         }
         more = more && link >= 0;

@@ -11,8 +11,7 @@ uint64_t Image::bytes() const {
   uint64_t b = meta.size() * sizeof(GroupMeta) + succ4.size() * 4 + cls.size() * 4 +
                sod.size() * 4;
   for (auto &m : memo)
-    b += (m.key.size() + m.beg.size() + m.nrep.size() + m.dir.size()) * 4 +
-         m.list.size() * 4;
+    b += (m.beg.size() + m.nrep.size() + m.sel.size() + m.list.size()) * 4;
   return b;
 }
 
@@ -55,6 +54,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
   img.num_groups = num_groups;
   img.memo_depth = memo_depth;
   for (int d = 0; d < kMaxMemoDepth; d++) img.memo[d].beg.push_back(0);
+  std::vector<uint32_t> key2, grp2;  // 2-gram and group of every depth-2 list
 
   for (int32_t g = 0; g < num_groups; g++) {
     const HostGroup &hg = groups[g];
@@ -145,6 +145,10 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     gm.sod_begin = (uint32_t)img.sod.size();
     for (int i = 0; i < hg.num_sod; i++) img.sod.push_back(tag[hg.sod[i]]);
     gm.sod_end = (uint32_t)img.sod.size();
+    for (int i = 0; i < hg.num_sod; i++) {
+      img.xsod.push_back((uint32_t)tag[hg.sod[i]]);
+      img.xsod.push_back((uint32_t)g);
+    }
 
     // memo depth 1: all-input states by byte (all 256 keys present)
     std::vector<std::vector<int32_t>> l1(256);
@@ -159,49 +163,45 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
       if (!l1[b].empty()) img.mask1[m1 + b / 32] |= 1u << (b % 32);
       push_list(img.memo[0], l1[b], tag);
     }
-    size_t m2 = img.mask2.size();
-    if (memo_depth >= 2) img.mask2.resize(m2 + 2048, 0u);
-    // deeper levels
+    // deeper levels, each list's successors split by the next byte
     std::vector<char> mark(n, 0);
     std::vector<std::vector<int32_t>> bucket(256);
     std::vector<std::vector<int32_t>> prev_lists;  // lists of previous depth
-    std::vector<uint32_t> prev_keys;
-    for (int b = 0; b < 256; b++) { prev_lists.push_back(l1[b]); prev_keys.push_back(b); }
+    for (int b = 0; b < 256; b++) prev_lists.push_back(l1[b]);
     for (int d = 2; d <= memo_depth; d++) {
       MemoLevel &m = img.memo[d - 1];
-      gm.memo_key_base[d - 2] = (uint32_t)m.key.size();
+      // depth 2: 2048 pairs per group; depth 3: 8 pairs per depth-2 key
+      const size_t sel0 = m.sel.size();
+      m.sel.resize(sel0 + prev_lists.size() * 16, 0u);
       std::vector<std::vector<int32_t>> cur_lists;
-      std::vector<uint32_t> cur_keys;
-      size_t dir0 = m.dir.size();
-      m.dir.resize(dir0 + 257, 0);
-      int last_first = -1;
       for (size_t li = 0; li < prev_lists.size(); li++) {
+        // key bits: depth 2 uses (first byte li, second byte b) inside the
+        // group's 65536-key block; depth 3 uses the 256 keys below list li
+        uint32_t *pairs = &m.sel[sel0 + li * 16];
         std::vector<int32_t> S;
         for (int32_t v : prev_lists[li])
           for (int32_t u : succ[v])
             if (!mark[u]) { mark[u] = 1; S.push_back(u); }
         for (int32_t u : S) mark[u] = 0;
-        if (S.empty()) continue;
         for (auto &bk : bucket) bk.clear();
         for (int32_t u : S)
           for (int b = 0; b < 256; b++)
             if (has_byte(hg.sets, u, b)) bucket[b].push_back(u);
-        for (int b = 0; b < 256; b++) {
-          if (bucket[b].empty()) continue;
-          uint32_t key = (prev_keys[li] << 8) | (uint32_t)b;
-          int first = (int)(key >> (8 * (d - 1)));
-          while (last_first < first) m.dir[dir0 + (++last_first)] = (uint32_t)(m.key.size() - gm.memo_key_base[d - 2]);
-          m.key.push_back(key);
-          if (d == 2) img.mask2[m2 + (key >> 5)] |= 1u << (key & 31);
-          cur_lists.push_back(bucket[b]);
-          cur_keys.push_back(key);
-          push_list(m, bucket[b], tag);
+        for (int w = 0; w < 8; w++) {
+          pairs[2 * w + 1] = (uint32_t)m.keys();
+          for (int b = w * 32; b < w * 32 + 32; b++) {
+            if (bucket[b].empty()) continue;
+            pairs[2 * w] |= 1u << (b & 31);
+            if (d == 2) {
+              key2.push_back(((uint32_t)li << 8) | (uint32_t)b);
+              grp2.push_back((uint32_t)g);
+            }
+            cur_lists.push_back(bucket[b]);
+            push_list(m, bucket[b], tag);
+          }
         }
       }
-      uint32_t nk = (uint32_t)(m.key.size() - gm.memo_key_base[d - 2]);
-      while (last_first < 256) m.dir[dir0 + (++last_first)] = nk;
       prev_lists.swap(cur_lists);
-      prev_keys.swap(cur_keys);
     }
     img.meta.push_back(gm);
     img.stat_edges.push_back((uint32_t)E);
@@ -211,6 +211,35 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     img.stat_wide.push_back(wide);
     img.stat_selfloop.push_back(selfloop);
     img.total_states += n;
+  }
+  // cross-group index: counting sort of the lists by byte / 2-gram
+  img.xr1.assign(257, 0);
+  img.xmask1.assign(8, 0);
+  for (int32_t g = 0; g < num_groups; g++)
+    for (int b = 0; b < 256; b++)
+      if (img.memo[0].nrep[(size_t)g * 256 + b]) img.xr1[b + 1]++;
+  for (int b = 0; b < 256; b++) {
+    if (img.xr1[b + 1]) img.xmask1[b / 32] |= 1u << (b % 32);
+    img.xr1[b + 1] += img.xr1[b];
+  }
+  img.xe1.resize(img.xr1[256]);
+  {
+    std::vector<uint32_t> at(img.xr1.begin(), img.xr1.end() - 1);
+    for (int32_t g = 0; g < num_groups; g++)
+      for (int b = 0; b < 256; b++)
+        if (img.memo[0].nrep[(size_t)g * 256 + b]) img.xe1[at[b]++] = (uint32_t)g * 256 + b;
+  }
+  if (memo_depth >= 2) {
+    img.xr2.assign(65537, 0);
+    for (uint32_t k : key2) img.xr2[k + 1]++;
+    for (int k = 0; k < 65536; k++) img.xr2[k + 1] += img.xr2[k];
+    img.xe2.resize((size_t)img.xr2[65536] * 2);
+    std::vector<uint32_t> at(img.xr2.begin(), img.xr2.end() - 1);
+    for (size_t i = 0; i < key2.size(); i++) {  // list index i (depth-2 lists in order)
+      const uint32_t slot = at[key2[i]]++;
+      img.xe2[2 * (size_t)slot] = (uint32_t)i;
+      img.xe2[2 * (size_t)slot + 1] = grp2[i];
+    }
   }
   for (auto &m : img.memo)
     if (m.list.size() >= (1ull << 31)) {

@@ -21,10 +21,10 @@
 //      d = 2: non-all-input successors of d=1 entries that match input[p+1]
 //      d = 3: ... of d=2 entries that match input[p+2]
 //    Each list is a set (no duplicates) with reporting states first.  d = 1
-//    is indexed directly by byte; d = 2, 3 keep sorted non-empty keys with a
-//    per-first-byte directory and a binary search inside the bucket.  This is
-//    the reference's prefix memoization (ngap.cu:2358-2513,
-//    precompute_table.cu:25-395) with set lists and a bucketed key index.
+//    is indexed directly by byte; d = 2, 3 through rank tables (one 8-byte
+//    load and a popcount per lookup, MemoLevel below).  This is the
+//    reference's prefix memoization (ngap.cu:2358-2513,
+//    precompute_table.cu:25-395) with set lists and rank-table indexes.
 #pragma once
 
 #include <cstdint>
@@ -58,15 +58,25 @@ struct GroupMeta {            // per group, read by every kernel (32 B)
   uint32_t sod_begin, sod_end;// start-of-data entries (tagged ids) in sod[]
   uint32_t num_states;
   uint32_t num_aas;
-  uint32_t memo_key_base[2];  // first key index for depth 2 and 3
+  uint32_t pad[2];
 };
 
+// Memo level d holds one list per d-gram key that has one.  Keys of a level
+// are numbered in (group, key) order; list k is list[beg[k], beg[k+1]) with
+// its nrep[k] reporting entries first.  Depth 1 is indexed by g*256 + byte
+// (all 256 keys present).  Depths 2 and 3 are found through rank tables of
+// (presence word, rank) pairs: the 32 keys x..x+31 share one pair whose word
+// has bit i set when key x+i has a list and whose rank is the index of the
+// first such key, so key k's index is rank + popcount(word & ((1<<i)-1)).
+//   depth 2: pair g*2048 + (2-gram >> 5)
+//   depth 3: pair k2*8 + (third byte >> 5), k2 = index of the first two bytes'
+//            depth-2 key (8 pairs per depth-2 key)
 struct MemoLevel {            // one depth for all groups
-  std::vector<uint32_t> key;   // sorted keys per group (d >= 2)
   std::vector<uint32_t> beg;   // [nkeys + 1] list offsets (global)
   std::vector<uint32_t> nrep;  // reporting entries at the head of each list
-  std::vector<uint32_t> dir;   // d >= 2: [groups*257] key-index ranges by first byte
+  std::vector<uint32_t> sel;   // d >= 2: rank table, (word, rank) pairs
   std::vector<int32_t> list;   // tagged state ids
+  uint64_t keys() const { return nrep.size(); }
 };
 
 struct Image {
@@ -77,10 +87,16 @@ struct Image {
   std::vector<uint32_t> cls;    // 8 per state
   std::vector<int32_t> sod;     // tagged SOD ids
   MemoLevel memo[kMaxMemoDepth];// memo[0] = depth 1 (always built)
-  // root filters: per group 8 words (byte b has a depth-1 list) and, when
-  // depth >= 2 is built, 2048 words (2-gram key has a depth-2 list)
+  // root filter: per group 8 words, bit b set when byte b has a depth-1 list
   std::vector<uint32_t> mask1;
-  std::vector<uint32_t> mask2;
+  // Cross-group root index (memo depth >= 2), for work items that scan one
+  // window for every group at once:
+  //   x1: byte b -> depth-1 lists with reporting entries, xe1[xr1[b]..xr1[b+1])
+  //       holding list indices g*256 + b; xmask1 marks bytes with any;
+  //   x2: 2-gram k -> depth-2 lists of every group, xe2 pairs (list index,
+  //       group) in xe2[2*xr2[k] .. 2*xr2[k+1]);
+  //   xsod: (tagged id, group) pairs of every group's start-of-data states.
+  std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod;
   // statistics (for the learned scheduler's features and reports)
   std::vector<uint32_t> stat_edges, stat_aas, stat_sod, stat_report,
       stat_wide, stat_selfloop;

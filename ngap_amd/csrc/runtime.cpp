@@ -122,11 +122,16 @@ KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_op
     P.img.beg[d] = eng->d_beg[d].as<uint32_t>();
     P.img.nrep[d] = eng->d_nrep[d].as<uint32_t>();
     P.img.list[d] = eng->d_list[d].as<int32_t>();
-    P.img.key[d] = eng->d_key[d].as<uint32_t>();
-    P.img.dir[d] = eng->d_dir[d].as<uint32_t>();
+    P.img.sel[d] = eng->d_sel[d].as<uint32_t>();
   }
   P.img.mask1 = eng->d_mask1.as<uint32_t>();
-  P.img.mask2 = eng->d_mask2.as<uint32_t>();
+  P.img.xr1 = eng->d_xr1.as<uint32_t>();
+  P.img.xe1 = eng->d_xe1.as<uint32_t>();
+  P.img.xmask1 = eng->d_xmask1.as<uint32_t>();
+  P.img.xr2 = eng->d_xr2.as<uint32_t>();
+  P.img.xe2 = eng->d_xe2.as<uint32_t>();
+  P.img.xsod = eng->d_xsod.as<uint32_t>();
+  P.img.n_xsod = (uint32_t)(eng->img.xsod.size() / 2);
   P.num_groups = (uint32_t)eng->img.num_groups;
   P.memo_depth = mp.memo_depth;
   P.is_dup = o.is_dup;
@@ -334,20 +339,57 @@ std::vector<Task> static_schedule(const ngap_engine *eng, uint32_t window, bool 
   return out;
 }
 
+// Static schedule of cross-group work items: every chunk cut into windows
+// that scan all groups at once; by default about four work items per wave.
+std::vector<Task> cross_schedule(const ngap_engine *eng, uint32_t window, bool is_dup,
+                                 uint32_t waves) {
+  const uint32_t S = eng->num_seg, L = (uint32_t)eng->chunk_len;
+  uint32_t win = window;
+  if (!win) {
+    const uint64_t per = (uint64_t)S * L / std::max<uint64_t>(1, 4ull * waves);
+    win = 256;
+    while ((uint64_t)win * 2 <= per && win < 65536) win *= 2;
+  }
+  if (win > L) win = L;
+  std::vector<Task> tasks;
+  for (uint32_t c = 0; c < S; c++)
+    for (uint32_t a = 0; a < L; a += win) {
+      const uint32_t b = (uint32_t)std::min<uint64_t>((uint64_t)a + win, L);
+      const uint32_t cs = c * L;
+      const uint32_t seed = (a == 0 && (c == 0 || is_dup)) ? kTaskSeedStart : 0u;
+      tasks.push_back({kAllGroups, cs, cs + L, cs + a, cs + b, seed});
+    }
+  return tasks;
+}
+
 // Host work items -> device tasks, checking that each (group, chunk) unit is
 // tiled exactly once.
-int host_tasks(const ngap_engine *eng, const ngap_run_opts &o, std::vector<Task> &out) {
+int host_tasks(const ngap_engine *eng, const ngap_run_opts &o, int memo_depth,
+               std::vector<Task> &out) {
   const uint32_t G = (uint32_t)eng->img.num_groups, S = eng->num_seg, L = (uint32_t)eng->chunk_len;
-  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per((size_t)G * S);
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per((size_t)G * S), all(S);
   out.clear();
   for (uint32_t i = 0; i < o.num_tasks; i++) {
     const ngap_task &t = o.tasks[i];
-    if (t.group >= G || t.chunk >= S || t.begin >= t.end || t.end > L)
+    const bool every = t.group == NGAP_ALL_GROUPS;
+    if ((!every && t.group >= G) || t.chunk >= S || t.begin >= t.end || t.end > L)
       return ngap_fail(NGAP_ERR_INVALID, "work item " + std::to_string(i) + " out of range");
-    per[(size_t)t.chunk * G + t.group].emplace_back(t.begin, t.end);
+    if (every && memo_depth < 2)
+      return ngap_fail(NGAP_ERR_INVALID, "all-group work items need precompute depth >= 2");
+    if (every) all[t.chunk].emplace_back(t.begin, t.end);
+    else per[(size_t)t.chunk * G + t.group].emplace_back(t.begin, t.end);
     const uint32_t cs = t.chunk * L;
     const uint32_t seed = (t.begin == 0 && (t.chunk == 0 || o.is_dup)) ? kTaskSeedStart : 0u;
     out.push_back({t.group, cs, cs + L, cs + t.begin, cs + t.end, seed});
+  }
+  for (uint32_t c = 0; c < S; c++) {
+    if (all[c].empty()) continue;
+    for (uint32_t g = 0; g < G; g++)
+      if (!per[(size_t)c * G + g].empty())
+        return ngap_fail(NGAP_ERR_INVALID,
+                         "a chunk with all-group work items takes no per-group work items");
+    per[(size_t)c * G].swap(all[c]);  // checked below like one group's tiling
+    for (uint32_t g = 1; g < G; g++) per[(size_t)c * G + g] = {{0u, L}};
   }
   for (auto &v : per) {
     std::sort(v.begin(), v.end());
@@ -438,6 +480,7 @@ void ngap_default_opts(ngap_run_opts *o) {
   o->add_aas_start = 0;
   o->add_aas_interval = 1;
   o->result_capacity = 54619400ull;
+  o->cross_groups = 1;
 }
 
 int ngap_create(int32_t device, ngap_engine **out) {
@@ -494,13 +537,17 @@ int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t nu
   HIP_TRY(eng->d_cls.upload(im.cls));
   HIP_TRY(eng->d_sod.upload(im.sod));
   HIP_TRY(eng->d_mask1.upload(im.mask1));
-  HIP_TRY(eng->d_mask2.upload(im.mask2));
+  HIP_TRY(eng->d_xr1.upload(im.xr1));
+  HIP_TRY(eng->d_xe1.upload(im.xe1));
+  HIP_TRY(eng->d_xmask1.upload(im.xmask1));
+  HIP_TRY(eng->d_xr2.upload(im.xr2));
+  HIP_TRY(eng->d_xe2.upload(im.xe2));
+  HIP_TRY(eng->d_xsod.upload(im.xsod));
   for (int d = 0; d < kMaxMemoDepth; d++) {
     HIP_TRY(eng->d_beg[d].upload(im.memo[d].beg));
     HIP_TRY(eng->d_nrep[d].upload(im.memo[d].nrep));
     HIP_TRY(eng->d_list[d].upload(im.memo[d].list));
-    HIP_TRY(eng->d_key[d].upload(im.memo[d].key));
-    HIP_TRY(eng->d_dir[d].upload(im.memo[d].dir));
+    HIP_TRY(eng->d_sel[d].upload(im.memo[d].sel));
   }
   eng->loaded = true;
   eng->ctx.have_run = false;
@@ -557,8 +604,14 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
     return run_plain(eng, ctx, P, o, st);
   std::vector<Task> tasks;
   if (o.tasks) {
-    int rc = host_tasks(eng, o, tasks);
+    int rc = host_tasks(eng, o, mp.memo_depth, tasks);
     if (rc) return rc;
+  } else if (mp.memo_depth >= 2 &&
+             (o.cross_groups == 2 || (o.cross_groups == 1 && eng->img.num_groups > 1))) {
+    uint32_t blocks = 0;
+    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks);
+    if (rc) return rc;
+    tasks = cross_schedule(eng, o.window, o.is_dup != 0, blocks * P.waves_per_block);
   } else {
     tasks = static_schedule(eng, o.window, o.is_dup != 0);
   }

@@ -80,10 +80,10 @@ struct ngap_engine {
   int num_cus = 0;
   ngapamd::Image img;
   bool loaded = false;
-  ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1, d_mask2;
+  ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;
   ngapamd::DevBuf d_beg[ngapamd::kMaxMemoDepth], d_nrep[ngapamd::kMaxMemoDepth],
-      d_list[ngapamd::kMaxMemoDepth], d_key[ngapamd::kMaxMemoDepth],
-      d_dir[ngapamd::kMaxMemoDepth];
+      d_list[ngapamd::kMaxMemoDepth], d_sel[ngapamd::kMaxMemoDepth];
+  ngapamd::DevBuf d_xr1, d_xe1, d_xmask1, d_xr2, d_xe2, d_xsod;
   ngapamd::DevBuf d_input;
   uint64_t chunk_len = 0;
   uint32_t num_seg = 0;

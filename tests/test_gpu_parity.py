@@ -76,6 +76,9 @@ def test_regex_patterns_multi_chunk(engine, seed):
                                   ("nonblockingpcgroups", "plain", 3, 1, 0),
                                   ("nonblockingr1groups", "batched", 1, 0, 8192),
                                   ("NAPgroups", "plain", 0, 0, 0)]]
+    # one group through the cross-group index (forced)
+    cfgs += [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=d,
+                  unique=1, cross_groups=2, window=w) for d, w in ((2, 0), (3, 777))]
     check_all(engine, [g], data, 1 << 14, None, cfgs)
 
 
@@ -86,6 +89,15 @@ def test_many_groups_with_dot_star(engine):
     data = synth.zipf_bytes(rng, 1 << 17, 1.1)
     cfgs = [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d, unique=1)
             for s in ("plain", "batched") for d in (1, 2, 3)]
+    # batched with one group per work item, and cross-group work items with
+    # forced windows
+    cfgs += [dict(algorithm=a, schedule="batched", precompute_depth=d, unique=u,
+                  cross_groups=x, window=w)
+             for a, d, u, x, w in [("nonblockingallgroups", 2, 1, 0, 0),
+                                   ("nonblockingallgroups", 3, 0, 0, 2048),
+                                   ("nonblockingallgroups", 2, 0, 1, 256),
+                                   ("nonblockingpcgroups", 3, 1, 1, 1000),
+                                   ("nonblockingallgroups", 3, 1, 2, 8192)]]
     check_all(engine, groups, data, 1 << 13, None, cfgs)
 
 
@@ -99,6 +111,9 @@ def test_class_repeat_chains(engine, seed):
     cfgs = [dict(algorithm=a, schedule=s, precompute_depth=d, unique=u)
             for a in ("nonblockinggroups", "nonblockingallgroups")
             for s in ("plain", "batched") for d in (1, 3) for u in (0, 1)]
+    cfgs += [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=d,
+                  unique=u, cross_groups=2, window=w)
+             for d, u, w in ((2, 0, 0), (3, 1, 0), (3, 0, 64))]
     check_all(engine, [g], data, 250, None, cfgs)
 
 
@@ -111,7 +126,9 @@ def test_start_of_data_chunks(engine, is_dup):
     data = bytes(r.choice(alpha) for _ in range(640))
     cfgs = [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d)
             for s in ("plain", "batched") for d in (1, 2)]
-    check_all(engine, [g], data, 64, None, cfgs, is_dup=is_dup)
+    cfgs += [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=d,
+                  cross_groups=2, window=w) for d, w in ((2, 16), (3, 0))]
+    check_all(engine, [g, g], data, 64, None, cfgs, is_dup=is_dup)
 
 
 @pytest.mark.parametrize("seed", range(3))
@@ -139,6 +156,8 @@ def test_wide_fanout_direct_api(engine):
     cfgs = [dict(algorithm=a, schedule=s, precompute_depth=d, unique=u)
             for a in ("nonblockinggroups", "nonblockingallgroups")
             for s in ("plain", "batched") for d in (1, 2, 3) for u in (0, 1)]
+    cfgs += [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=d,
+                  unique=u, cross_groups=2) for d in (2, 3) for u in (0, 1)]
     check_all(engine, [g], data, 600, None, cfgs)
 
 
@@ -146,7 +165,12 @@ def test_chunk_of_one_byte_and_no_reports(engine):
     g = make_group(2, [(0, 1)], [[ord("a")], [ord("b")]], all_input=[0], reporting=[1])
     check_all(engine, [g], b"ab" * 64, 1, None,
               [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d)
-               for s in ("plain", "batched") for d in (1, 3)])
+               for s in ("plain", "batched") for d in (1, 3)] +
+              [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=d,
+                    cross_groups=2) for d in (2, 3)])
+    check_all(engine, [g, g], b"ab" * 64, 2, None,
+              [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=d,
+                    cross_groups=2) for d in (2, 3)])
     check_all(engine, [g], b"zz" * 64, 16, None,
               [dict(algorithm="nonblockingallgroups", schedule="plain")])
 

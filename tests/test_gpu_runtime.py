@@ -73,6 +73,41 @@ def test_host_work_items(engine):
         engine.run(algorithm="nonblockingallgroups", schedule="batched", tasks=tasks + tasks[:1])
 
 
+def test_all_group_work_items(engine):
+    """Chunks tiled by all-group work items (roots of every group from the
+    cross-group index) beside chunks tiled per group, in one launch."""
+    groups, data = _workload(seed=23)
+    L = 1 << 13
+    S = len(data) // L
+    want, _ = oracle.run(groups, data, L)
+    engine.load_groups(groups, 3)
+    engine.set_input(data, L)
+    rng = np.random.default_rng(1)
+    ALL = -1
+    tasks = []
+    for c in range(S):
+        if c % 2 == 0:
+            cuts = sorted(set([0, L] + list(rng.integers(1, L, rng.integers(0, 6)))))
+            tasks += [(ALL, c, a, b) for a, b in zip(cuts, cuts[1:])]
+        else:
+            tasks += [(g, c, 0, L) for g in range(len(groups))]
+    tasks = [tasks[i] for i in rng.permutation(len(tasks))]
+    for d in (2, 3):
+        say("all-group work items, depth", d)
+        st = engine.run(algorithm="nonblockingallgroups", schedule="batched",
+                        precompute_depth=d, unique=True, tasks=tasks)
+        assert st["num_tasks"] == len(tasks)
+        assert np.array_equal(engine.reports(), want)
+    done = engine.tasks()
+    assert int(np.sum(done[:, 0] == 0xFFFFFFFF)) == sum(1 for t in tasks if t[0] == ALL)
+    say("all-group work items rejected")
+    with pytest.raises(NgapError):   # depth 1 has no cross-group index
+        engine.run(algorithm="nonblockinggroups", schedule="batched", tasks=tasks)
+    with pytest.raises(NgapError):   # a chunk mixing both kinds
+        engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
+                   tasks=tasks + [(0, 0, 0, L)])
+
+
 def test_overflow_fallback_restores_exact_reports(engine):
     n = 1 + 4 + 16 + 64
     edges = [(0, 1 + i) for i in range(4)] + [(1 + i, 5 + 4 * i + j) for i in range(4) for j in range(4)]
