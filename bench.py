@@ -164,7 +164,10 @@ def learned_comparison(depth, steps, workload_name):
     pred = m.predict_pairs(hist, stats, all_pairs)
     cost = learned.predicted_cost(pred).reshape(len(groups), S * K)
     blocks_waves = 256 * 4 * 4
-    tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, blocks_waves)
+    if depth >= 2:   # all-group work items (cross-group root index)
+        tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, blocks_waves)
+    else:
+        tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, blocks_waves)
     spill = learned.spill_capacity(pred, tcost)
     sched_s = time.time() - t2
     opts = eng.options(schedule="batched", tasks=tasks, spill_capacity=spill, fallback=True,

@@ -135,6 +135,17 @@ int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
 
 int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats);
 
+/* Autotuner of the batched engine (the reference picks these knobs by hand
+ * per application, code/scripts/configs/exec_config_ngap_groups_best): times
+ * candidate work-item windows, persistent grid sizes and worklist spill
+ * capacities on the loaded input by coordinate descent (at most max_runs
+ * timed runs) and writes the fastest into *best: a copy of *base, batched,
+ * with fallback on, window / persistent_blocks / spill_capacity set.  Report
+ * sets do not depend on these knobs; the chunk length is part of the input's
+ * meaning (matches do not cross chunks) and is not tuned. */
+int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
+                  ngap_run_opts *best, double *best_ms);
+
 /* Sorted, de-duplicated reports of the last run, reference encoding
  * ((state | group << 22) << 32) | offset.  *n_out gets the total count; at
  * most cap entries are written (out may be NULL to query the count). */

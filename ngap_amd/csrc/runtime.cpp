@@ -618,6 +618,70 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   return run_tasks(eng, ctx, P, tasks, o, st);
 }
 
+int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
+                  ngap_run_opts *best, double *best_ms) {
+  if (!eng || !base || !best) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  if (!eng->loaded || !eng->have_input) return ngap_fail(NGAP_ERR_STATE, "no groups or input");
+  HIP_TRY(hipSetDevice(eng->device));
+  ngap_run_opts o = *base;
+  o.schedule = NGAP_SCHED_BATCHED;
+  o.fallback = 1;
+  o.tasks = nullptr;
+  o.num_tasks = 0;
+  int per_cu = 0;
+  HIP_TRY(batched_occupancy(&per_cu, lds_bytes()));
+  const uint32_t full = (uint32_t)std::max(1, per_cu) * (uint32_t)eng->num_cus;
+  const uint32_t L = (uint32_t)eng->chunk_len;
+  int runs = 0;
+  double best_t = 1e300;
+  ngap_run_opts cur = o;
+  // time one setting: the faster of two runs (kernel plus fallback time)
+  auto timeit = [&](const ngap_run_opts &c, double &t) -> int {
+    t = 1e300;
+    for (int r = 0; r < 2; r++) {
+      ngap_run_stats st;
+      int rc = ngap_run(eng, &c, &st);
+      if (rc) return rc;
+      t = std::min(t, st.kernel_ms + st.fallback_ms);
+    }
+    runs++;
+    return NGAP_OK;
+  };
+  int rc = timeit(cur, best_t);
+  if (rc) return rc;
+  std::vector<uint32_t> windows = {256, 512, 1024, 2048, 4096, 8192, 16384};
+  std::vector<uint32_t> grids = {full / 4, full / 2, (full * 3) / 4, full};
+  std::vector<uint32_t> spills = {8192, 16384, 262144};
+  for (int axis = 0; axis < 3; axis++) {
+    const std::vector<uint32_t> &vals = axis == 0 ? windows : axis == 1 ? grids : spills;
+    ngap_run_opts pick = cur;
+    for (uint32_t v : vals) {
+      if (runs >= max_runs) break;
+      ngap_run_opts c = cur;
+      if (axis == 0) {
+        if (v > L) continue;
+        c.window = v;
+      } else if (axis == 1) {
+        if (v == 0) continue;
+        c.persistent_blocks = v;
+      } else {
+        c.spill_capacity = v;
+      }
+      double t = 0;
+      rc = timeit(c, t);
+      if (rc) return rc;
+      if (t < best_t) {
+        best_t = t;
+        pick = c;
+      }
+    }
+    cur = pick;
+  }
+  *best = cur;
+  if (best_ms) *best_ms = best_t;
+  return NGAP_OK;
+}
+
 int ngap_get_reports(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out) {
   if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
   RunCtx &ctx = eng->ctx;

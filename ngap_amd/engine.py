@@ -92,6 +92,16 @@ class Engine:
             o._keep = arr  # keep the array alive with the options
         return o
 
+    def autotune(self, max_runs=16, **kw):
+        """Batched-engine knobs (window, persistent blocks, spill capacity)
+        timed on the loaded input; returns (options, best kernel ms)."""
+        base = self.options(**kw)
+        best = _lib.RunOpts()
+        ms = C.c_double()
+        check(_lib.lib().ngap_autotune(self._h, C.byref(base), int(max_runs), C.byref(best),
+                                       C.byref(ms)))
+        return best, ms.value
+
     def run(self, opts=None, **kw):
         """One launch.  Returns a dict of stats; raises NgapError on failure
         (overflow included: ``err.code == NGAP_ERR_OVERFLOW``)."""

@@ -115,6 +115,46 @@ def schedule_from_costs(cost, num_groups, num_seg, chunk, waves, window=WINDOW,
     return np.asarray(tasks, np.uint32)[order], np.asarray(tcost)[order]
 
 
+ALL_GROUPS = 0xFFFFFFFF
+
+
+def cross_schedule_from_costs(cost, num_seg, chunk, waves, window=WINDOW, per_wave=4,
+                              min_piece=256):
+    """All-group work items (every group's roots in one window, precompute
+    depth >= 2) shaped by per-(group, window) costs: a window's cost is the
+    sum over groups; a window above the target total / (waves * per_wave) is
+    cut into equal pieces (at least min_piece bytes), consecutive windows
+    below it are merged.  Returns work items ordered by decreasing cost and
+    their costs."""
+    K = windows_per_chunk(chunk, window)
+    wcost = np.asarray(cost, np.float64).reshape(-1, num_seg * K).sum(0)
+    target = max(wcost.sum() / max(1, waves * per_wave), 1e-9)
+    tasks, tcost = [], []
+    for c in range(num_seg):
+        a, acc = 0, 0.0
+        for k in range(K):
+            lo, hi = k * window, min((k + 1) * window, chunk)
+            w = wcost[c * K + k]
+            if w >= target:
+                if acc > 0:
+                    tasks.append((ALL_GROUPS, c, a, lo))
+                    tcost.append(acc)
+                n = int(min(np.ceil(w / target), max(1, (hi - lo) // min_piece)))
+                cuts = [lo + (hi - lo) * i // n for i in range(n + 1)]
+                for x, y in zip(cuts, cuts[1:]):
+                    tasks.append((ALL_GROUPS, c, x, y))
+                    tcost.append(w / n)
+                a, acc = hi, 0.0
+                continue
+            acc += w
+            if acc >= target or k == K - 1:
+                tasks.append((ALL_GROUPS, c, a, hi))
+                tcost.append(acc)
+                a, acc = hi, 0.0
+    order = np.argsort(-np.asarray(tcost), kind="stable")
+    return np.asarray(tasks, np.uint32)[order], np.asarray(tcost)[order]
+
+
 def predicted_cost(pred_log_items, window=WINDOW):
     return np.expm1(np.maximum(pred_log_items, 0.0)) + SCAN_COST * window
 

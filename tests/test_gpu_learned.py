@@ -97,4 +97,11 @@ def test_learned_schedule_end_to_end(engine):
                     fallback=True)
     assert np.array_equal(engine.reports(), want)
     assert st["num_tasks"] == len(tasks)
+    ctasks, ccost = learned.cross_schedule_from_costs(cost, S, chunk, waves=1024)
+    say("learned all-group schedule run")
+    st = engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
+                    unique=True, tasks=ctasks, spill_capacity=learned.spill_capacity(pred, ccost),
+                    fallback=True)
+    assert np.array_equal(engine.reports(), want)
+    assert st["num_tasks"] == len(ctasks)
     m.close()

@@ -108,6 +108,21 @@ def test_all_group_work_items(engine):
                    tasks=tasks + [(0, 0, 0, L)])
 
 
+def test_autotuner_picks_knobs_not_results(engine):
+    groups, data = _workload(seed=24)
+    L = 1 << 13
+    want, _ = oracle.run(groups, data, L)
+    engine.load_groups(groups, 2)
+    engine.set_input(data, L)
+    say("autotune")
+    best, ms = engine.autotune(max_runs=10, algorithm="nonblockingallgroups",
+                               precompute_depth=2, unique=True)
+    assert ms > 0 and best.schedule == 1 and best.fallback == 1
+    assert best.window in (0, 256, 512, 1024, 2048, 4096, 8192)
+    engine.run(best)
+    assert np.array_equal(engine.reports(), want)
+
+
 def test_overflow_fallback_restores_exact_reports(engine):
     n = 1 + 4 + 16 + 64
     edges = [(0, 1 + i) for i in range(4)] + [(1 + i, 5 + 4 * i + j) for i in range(4) for j in range(4)]

@@ -67,6 +67,23 @@ def test_schedule_merges_cheap_windows_and_tiles_exactly():
     assert (heavy[:, 3] - heavy[:, 2]).mean() < (light[:, 3] - light[:, 2]).mean()
 
 
+def test_cross_schedule_splits_heavy_windows_and_tiles_exactly():
+    rng = np.random.default_rng(5)
+    G, S, L, W = 5, 3, 4096 * 8, 4096
+    cost = rng.exponential(1.0, (G, S * 8))
+    cost[:, 5] *= 200  # one heavy window in chunk 0
+    tasks, tc = learned.cross_schedule_from_costs(cost, S, L, waves=4, window=W)
+    assert np.all(tasks[:, 0] == learned.ALL_GROUPS)
+    assert np.all(np.diff(tc) <= 1e-9) and np.isclose(tc.sum(), cost.sum())
+    for c in range(S):
+        rows = tasks[tasks[:, 1] == c]
+        rows = rows[np.argsort(rows[:, 2])]
+        assert rows[0, 2] == 0 and rows[-1, 3] == L
+        assert np.all(rows[1:, 2] == rows[:-1, 3])
+    heavy = tasks[(tasks[:, 1] == 0) & (tasks[:, 2] >= 5 * W) & (tasks[:, 3] <= 6 * W)]
+    assert len(heavy) > 1 and np.all(heavy[:, 3] - heavy[:, 2] >= 256)
+
+
 def test_group_stats():
     g = make_group(3, [(0, 0), (0, 1), (1, 2)], [list(range(256)), [1], [2, 3]],
                    all_input=[0], reporting=[2])
