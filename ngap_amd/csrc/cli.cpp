@@ -438,30 +438,39 @@ int main(int argc, char **argv) {
   };
 
   if (o.tuning) {
-    // --tuning (ngap.cu:1995-2056): a grid over this engine's knobs, keeping
-    // the best validated throughput.
-    const int windows[] = {1024, 4096, 16384};
-    const char *scheds[] = {"plain", "batched"};
+    // --tuning (ngap.cu:1995-2056): the plain schedule, then the batched
+    // engine with the knobs the autotuner picks, keeping the best validated
+    // throughput.
     double best = -1;
     std::string best_s;
     auto tt0 = std::chrono::high_resolution_clock::now();
-    for (const char *sc : scheds)
-      for (int w : windows) {
-        if (std::string(sc) == "plain" && w != windows[0]) continue;
-        ngap_run_opts t = ro;
-        t.schedule = std::string(sc) == "plain" ? NGAP_SCHED_PLAIN : NGAP_SCHED_BATCHED;
-        t.window = w;
-        printf("[Tuning] Try choice: \n   schedule=%s, window=%d\n", sc, w);
-        bool pass = false;
-        double thr = one_run(t, pass);
-        if (pass && thr > best) {
-          best = thr;
-          best_s = std::string("schedule=") + sc + ", window=" + std::to_string(w);
-          printf("[Tuning] Update best choice: \n   %s max_throughput=%f\n", best_s.c_str(), best);
-        } else {
-          printf("[Tuning] Keep best choice: \n   %s max_throughput=%f\n", best_s.c_str(), best);
+    for (int choice = 0; choice < 2; choice++) {
+      ngap_run_opts t = ro;
+      std::string desc;
+      if (choice == 0) {
+        t.schedule = NGAP_SCHED_PLAIN;
+        desc = "schedule=plain";
+      } else {
+        double ms = 0;
+        if (ngap_autotune(eng, &ro, 24, &t, &ms) != NGAP_OK) {
+          printf("[Tuning] autotuner failed: %s\n", ngap_last_error());
+          continue;
         }
+        desc = "schedule=batched, window=" + std::to_string(t.window) +
+               ", persistent_blocks=" + std::to_string(t.persistent_blocks) +
+               ", spill_capacity=" + std::to_string(t.spill_capacity);
       }
+      printf("[Tuning] Try choice: \n   %s\n", desc.c_str());
+      bool pass = false;
+      double thr = one_run(t, pass);
+      if (pass && thr > best) {
+        best = thr;
+        best_s = desc;
+        printf("[Tuning] Update best choice: \n   %s max_throughput=%f\n", best_s.c_str(), best);
+      } else {
+        printf("[Tuning] Keep best choice: \n   %s max_throughput=%f\n", best_s.c_str(), best);
+      }
+    }
     auto tt1 = std::chrono::high_resolution_clock::now();
     std::cout << "[Tuning] Total time: "
               << std::chrono::duration_cast<std::chrono::microseconds>(tt1 - tt0).count() / 1e6
