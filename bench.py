@@ -68,7 +68,8 @@ def sample_stats(groups, data, chunk, depth, n_units=4, sample_len=16384, thread
     unit_bytes = sample.size * len(groups)
     per_byte = oracle.algorithmic_bytes(st, unit_bytes) / sample.size
     per_byte_ref = oracle.algorithmic_bytes_reference_model(st, unit_bytes) / sample.size
-    return per_byte, per_byte_ref, st, sample.size, dt, len(picks)
+    per_byte_cross = oracle.algorithmic_bytes_cross(st, sample.size, depth) / sample.size
+    return per_byte, per_byte_ref, per_byte_cross, st, sample.size, dt, len(picks)
 
 
 def cpu_baseline(groups, data, chunk, budget_s=15.0):
@@ -254,7 +255,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="regex20k")
     ap.add_argument("--depth", type=int, default=2)
-    ap.add_argument("--window", type=int, default=4096)
+    ap.add_argument("--window", type=int, default=0,
+                    help="batched work-item positions, 0 = the engine's choice")
+    ap.add_argument("--cross-groups", type=int, default=1, choices=[0, 1, 2],
+                    help="batched static schedule: all-group work items (1 = when several groups)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--only", choices=["plain", "batched"], default=None)
@@ -288,9 +292,10 @@ def main():
     log(f"loaded ({time.time() - t:.1f} s)")
 
     common = dict(algorithm="nonblockingallgroups", precompute_depth=a.depth, unique=True)
+    cross = a.depth >= 2 and (a.cross_groups == 2 or (a.cross_groups == 1 and len(groups) > 1))
     res = {}
     for sched in (["plain", "batched"] if a.only is None else [a.only]):
-        opts = eng.options(schedule=sched, window=a.window, **common)
+        opts = eng.options(schedule=sched, window=a.window, cross_groups=a.cross_groups, **common)
         wall, kern, st = timed(eng, opts, a.steps, a.warmup)
         res[sched] = dict(wall_ms=wall, kernel_ms=kern, stats=st, reports=eng.reports())
         log(f"{sched}: {wall:.3f} ms/step wall, {kern:.3f} ms kernel, "
@@ -303,8 +308,10 @@ def main():
     oracle_check = None
     per_byte = None
     if not a.no_check:
-        per_byte, per_byte_ref, st, nbytes, dt, nunits = sample_stats(groups, data, chunk,
-                                                                        max(1, a.depth))
+        per_byte, per_byte_ref, per_byte_cross, st, nbytes, dt, nunits = sample_stats(
+            groups, data, chunk, max(1, a.depth))
+        if main_s == "batched" and cross:
+            per_byte = per_byte_cross
         log(f"oracle sample: {nbytes} bytes over {nunits} chunks in {dt:.1f} s, "
             f"{per_byte:.2f} algorithmic bytes per input byte at memo depth {a.depth} "
             f"({per_byte_ref:.1f} for the reference algorithm); {st}")
@@ -332,6 +339,7 @@ def main():
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
                         kernel="k_batched" if main_s == "batched" else "k_plain",
+                        work_items="all-group" if main_s == "batched" and cross else "per-group",
                         algorithmic_bytes_per_input_byte=round(per_byte, 3),
                         reference_model_bytes_per_input_byte=round(per_byte_ref, 2))
     out = {
@@ -342,7 +350,8 @@ def main():
         "config": {"workload": a.workload, "groups": len(groups),
                    "states": int(sum(g.num_states for g in groups)), "input_bytes": int(total),
                    "chunk": int(chunk), "algorithm": "nonblockingallgroups",
-                   "precompute_depth": a.depth, "window": a.window},
+                   "precompute_depth": a.depth, "window": a.window,
+                   "all_group_work_items": bool(cross)},
         "kernel_ms": round(kern, 4), "kernel_mbps": round(total / (kern * 1e-3) / 1e6, 3),
         "plain_mbps": round(total / (res["plain"]["wall_ms"] * 1e-3) / 1e6, 3) if "plain" in res else None,
         "batched_mbps": round(total / (res["batched"]["wall_ms"] * 1e-3) / 1e6, 3) if "batched" in res else None,

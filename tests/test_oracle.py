@@ -129,6 +129,8 @@ def test_memo_work_counts_on_a_closed_form(k):
     assert st["expanded_tests"] == st["expanded_items"]
     assert st["reports"] == n
     assert st["root_positions"] == n               # every 'a'
+    if k >= 2:                                     # (counted for memo depth >= 2)
+        assert st["level2_lists"] == n             # every 'b' (one step below a root)
 
 
 def test_memo_depth_restarts_at_all_input_successors():
@@ -144,3 +146,4 @@ def test_memo_depth_restarts_at_all_input_successors():
     assert st["memo_list_items"] == 29            # each such 'a' at depth 1
     assert st["expanded_items"] == 29             # each expands to 'b'
     assert st["reports"] == 29
+    assert st["level2_lists"] == 29               # positions of those 'a's
