@@ -1,16 +1,25 @@
-"""Benchmark: input MB/s of the non-blocking automata engine on one MI355X.
+"""Benchmark: input MB/s of the non-blocking automata engine on one MI355X,
+batched + learned schedule against the plain engine, bit-exact reports.
 
-Default workload (regex20k): 20,000 random regex patterns (~474k states, one
-group, all components rooted at all-input states), 16 MiB of Zipf(1.1) input
-in 256 KiB chunks.  A step is one launch over the whole input (plain schedule:
-one workgroup per (group, chunk) unit; batched: persistent waves over 4 KiB
-work items), inputs resident in HBM, reports kept on the device.
+Default workload (learned): 1,024 automata, log-uniform 50..50,000 states (20 %
+with a leading dot-star), 8 MiB in 64 KiB chunks whose byte distribution
+drifts between Zipf s = 0.8 and 1.6, with pattern instances planted at 1e-4
+per byte.  Other workloads: batch1024 (same automata, Zipf(1.1) input),
+regex20k (20,000 patterns in one group, 16 MiB Zipf(1.1), 256 KiB chunks),
+literal64.
 
-Printed JSON (one line): value = input MB/s of the batched engine (wall clock
-over the timed steps, each step = counter reset + matching launch + overflow
-check), plus the plain engine's MB/s, a bit-exactness check (batched == plain
-in full, both == oracle on sampled chunks), the roofline of the dominant
-kernel and the CPU baseline (the oracle timed on host cores).
+A step is one pass of the engine over the whole input with inputs resident in
+HBM: counter reset + matching launch + overflow check (+ fallback re-runs).
+Engines timed on the same input: plain (one workgroup per (group, chunk)
+unit), batched with the static schedule (hipGraph replay), batched with the
+autotuned knobs, and batched with the learned schedule (the frontier-size MLP
+trained on traces of a second input from the same generator, different seed).
+
+Printed JSON (one line): value = input MB/s of the batched + learned engine
+(wall clock over the timed steps), the other engines' MB/s and speedups, the
+report-set checks, prediction MAE and overflow-fallback rate, the roofline of
+the dominant kernel, the CPU baseline (the oracle on host cores over a bounded
+sample), and the serving latencies.
 """
 import argparse
 import ctypes as C
@@ -40,18 +49,24 @@ def hip_sync():
 
 
 def workload(name):
+    """(groups, test input, training input, chunk): the training input comes
+    from the same generator with another seed (traces for the MLP)."""
     from ngap_amd import synth
     if name == "regex20k":
         groups, data, chunk = synth.regex20k()
+        train = synth.zipf_bytes(np.random.default_rng(3001), len(data), 1.1)
     elif name == "literal64":
         groups, data, chunk = synth.literal64()
+        train = synth.acgt_bytes(np.random.default_rng(3000), len(data))
     elif name == "batch1024":
         groups, data, chunk = synth.batch1024()
+        train = synth.zipf_bytes(np.random.default_rng(3002), len(data), 1.1)
     elif name == "learned":
         groups, data, chunk = synth.learned()
+        train = synth.drifting_input(np.random.default_rng(3003), len(data), chunk, 1e-4, groups)
     else:
         raise SystemExit(f"unknown workload {name}")
-    return groups, data, chunk
+    return groups, data, train, chunk
 
 
 def sample_stats(groups, data, chunk, depth, n_units=4, sample_len=16384, threads=0):
@@ -110,32 +125,15 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
-def learned_comparison(depth, steps, workload_name):
-    """batch1024 / learned workloads: plain (one workgroup per unit) vs the
-    batched engine with its static schedule vs the batched engine with the
-    MLP-shaped schedule; MLP trained on traces of an input with a different
-    seed.  Returns a dict (MB/s, speedups, MAE, overflow rate, bit-exactness)."""
-    from ngap_amd import Engine, learned, synth
+def learned_schedule(eng, groups, train, test, chunk, depth, waves=4096):
+    """Trains the frontier-size MLP on traces of `train` (at most 10^6
+    (group, 4 KiB window) samples) and shapes the batched schedule of `test`
+    from its predictions.  Leaves `test` loaded.  Returns (options, info)."""
+    from ngap_amd import learned
     from ngap_amd.mlp import MLP
-    t0 = time.time()
-    rng = np.random.default_rng(1002 if workload_name == "batch1024" else 1003)
-    groups = synth.batch_groups(rng, 1024)
-    chunk = 64 << 10
-    n = 8 << 20
-    if workload_name == "batch1024":
-        test = synth.zipf_bytes(np.random.default_rng(2002), n, 1.1)
-        train = synth.zipf_bytes(np.random.default_rng(3002), n, 1.1)
-    else:
-        test = synth.drifting_input(np.random.default_rng(2003), n, chunk, 1e-4, groups)
-        train = synth.drifting_input(np.random.default_rng(3003), n, chunk, 1e-4, groups)
-    log(f"{workload_name}: {sum(g.num_states for g in groups)} states, generated in "
-        f"{time.time() - t0:.1f} s")
-    eng = Engine(0)
-    eng.load_groups(groups, max(depth, 1))
     stats = learned.group_stats(groups)
-    # traces of the training input -> at most 10^6 samples
     eng.set_input(train, chunk)
-    t1 = time.time()
+    t0 = time.time()
     pairs, y = learned.collect_traces(eng, train, chunk, len(groups), depth)
     x = learned.features(learned.window_hist(train, chunk), stats, pairs)
     if x.shape[0] > 1000000:
@@ -143,59 +141,93 @@ def learned_comparison(depth, steps, workload_name):
         x, y = x[keep], y[keep]
     m = MLP(0, seed=7)
     loss = m.train(x, y, epochs=4, batch=4096, lr=2e-3)
-    log(f"trained on {x.shape[0]} traces in {time.time() - t1:.1f} s, loss {loss:.4f}")
-    # test input
+    t_train = time.time() - t0
     eng.set_input(test, chunk)
-    total = len(test)
-    common = dict(algorithm="nonblockingallgroups", precompute_depth=depth, unique=True)
-    out = {"workload": workload_name, "groups": len(groups),
-           "states": int(sum(g.num_states for g in groups)), "input_bytes": total,
-           "chunk": chunk, "train_traces": int(x.shape[0]), "train_loss": loss}
-    res = {}
-    wall, kern, st = timed(eng, eng.options(schedule="plain", **common), steps, 1)
-    res["plain"] = (wall, eng.reports())
-    wall, kern, st = timed(eng, eng.options(schedule="batched", **common), steps, 1)
-    res["static"] = (wall, eng.reports())
-    t2 = time.time()
-    S = total // chunk
+    t1 = time.time()
+    S = len(test) // chunk
     K = learned.windows_per_chunk(chunk)
     all_pairs = np.stack(np.meshgrid(np.arange(len(groups)), np.arange(S * K), indexing="ij"),
                          -1).reshape(-1, 2).astype(np.uint32)
     hist = learned.window_hist(test, chunk)
     pred = m.predict_pairs(hist, stats, all_pairs)
     cost = learned.predicted_cost(pred).reshape(len(groups), S * K)
-    blocks_waves = 256 * 4 * 4
-    if depth >= 2:   # all-group work items (cross-group root index)
-        tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, blocks_waves)
+    if depth >= 2 and len(groups) > 1:   # all-group work items
+        tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, waves)
     else:
-        tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, blocks_waves)
+        tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves)
     spill = learned.spill_capacity(pred, tcost)
-    sched_s = time.time() - t2
-    opts = eng.options(schedule="batched", tasks=tasks, spill_capacity=spill, fallback=True,
-                       use_graph=True, **common)
-    wall, kern, st = timed(eng, opts, steps, 1)
-    res["learned"] = (wall, eng.reports())
-    out["learned_fallback_tasks"] = int(st["fallback_tasks"])
-    out["learned_overflow_rate"] = st["fallback_tasks"] / max(1, st["num_tasks"])
-    out["learned_tasks"] = int(st["num_tasks"])
-    out["learned_spill_capacity"] = spill
-    out["schedule_host_s"] = sched_s
-    # prediction quality on the test input
+    t_sched = time.time() - t1
+    # prediction quality on the test input's own traces
     tp, ty = learned.collect_traces(eng, test, chunk, len(groups), depth)
     tpred = m.predict_pairs(hist, stats, tp)
-    out["mae_log_items"] = float(np.mean(np.abs(tpred - ty)))
-    out["mae_constant_predictor"] = float(np.mean(np.abs(np.mean(y) - ty)))
-    for k, (wall, _) in res.items():
-        out[f"{k}_mbps"] = total / (wall * 1e-3) / 1e6
-    out["speedup_learned_vs_plain"] = res["plain"][0] / res["learned"][0]
-    out["speedup_static_vs_plain"] = res["plain"][0] / res["static"][0]
-    out["bit_exact"] = bool(np.array_equal(res["plain"][1], res["static"][1]) and
-                            np.array_equal(res["plain"][1], res["learned"][1]))
-    out["reports"] = int(res["plain"][1].size)
-    log(f"{workload_name}: {out}")
+    info = {"train_traces": int(x.shape[0]), "train_loss": round(float(loss), 5),
+            "train_s": round(t_train, 2), "schedule_host_s": round(t_sched, 3),
+            "tasks": int(len(tasks)), "spill_capacity": int(spill),
+            "mae_log_items": round(float(np.mean(np.abs(tpred - ty))), 4),
+            "mae_constant_predictor": round(float(np.mean(np.abs(np.mean(y) - ty))), 4)}
     m.close()
+    opts = eng.options(schedule="batched", tasks=tasks, spill_capacity=spill, fallback=True,
+                       use_graph=True, algorithm="nonblockingallgroups",
+                       precompute_depth=depth, unique=True)
+    return opts, info
+
+
+def compare(name, depth, steps, warmup, learned_on=True, tune=True):
+    """All engines on one workload; returns (result dict, per-engine runs,
+    groups, test input, chunk)."""
+    from ngap_amd import Engine
+    t = time.time()
+    groups, test, train, chunk = workload(name)
+    total = (len(test) // chunk) * chunk
+    log(f"workload {name}: {len(groups)} groups, {sum(g.num_states for g in groups)} states, "
+        f"{total} bytes, chunk {chunk} ({time.time() - t:.1f} s)")
+    eng = Engine(0)
+    eng.load_groups(groups, max_precompute_depth=max(depth, 1))
+    common = dict(algorithm="nonblockingallgroups", precompute_depth=depth, unique=True)
+    runs = {}
+
+    def run(label, opts):
+        wall, kern, st = timed(eng, opts, steps, warmup)
+        runs[label] = dict(wall_ms=wall, kernel_ms=kern, stats=st, reports=eng.reports())
+        log(f"{name} {label}: {wall:.3f} ms/step wall, {kern:.3f} ms kernel, "
+            f"{total / wall / 1e3:.1f} MB/s, items {st['items']}, tasks {st['num_tasks']}, "
+            f"fallback {st['fallback_tasks']}, reports {runs[label]['reports'].size}")
+
+    info = None
+    if learned_on:
+        lopts, info = learned_schedule(eng, groups, train, test, chunk, max(depth, 1))
+        log(f"{name} learned schedule: {info}")
+    else:
+        eng.set_input(test, chunk)
+    run("plain", eng.options(schedule="plain", **common))
+    run("static", eng.options(schedule="batched", use_graph=True, **common))
+    if tune:
+        topts, tms = eng.autotune(max_runs=12, **common)
+        topts.use_graph = 1
+        run("tuned", topts)
+        tuned_knobs = dict(window=int(topts.window), persistent_blocks=int(topts.persistent_blocks),
+                           spill_capacity=int(topts.spill_capacity))
+    if learned_on:
+        run("learned", lopts)
     eng.close()
-    return out
+    ref = runs["plain"]["reports"]
+    res = {"workload": name, "groups": len(groups),
+           "states": int(sum(g.num_states for g in groups)), "input_bytes": int(total),
+           "chunk": int(chunk), "reports": int(ref.size),
+           "bit_exact_vs_plain": all(bool(np.array_equal(r["reports"], ref))
+                                     for r in runs.values())}
+    for k, r in runs.items():
+        res[f"{k}_mbps"] = round(total / (r["wall_ms"] * 1e-3) / 1e6, 3)
+        res[f"{k}_kernel_ms"] = round(r["kernel_ms"], 4)
+        if k != "plain":
+            res[f"speedup_{k}_vs_plain"] = round(runs["plain"]["wall_ms"] / r["wall_ms"], 3)
+    if tune:
+        res["tuned_knobs"] = tuned_knobs
+    if info:
+        st = runs["learned"]["stats"]
+        res["learned"] = dict(info, fallback_tasks=int(st["fallback_tasks"]),
+                              overflow_rate=st["fallback_tasks"] / max(1, st["num_tasks"]))
+    return res, runs, groups, test, chunk
 
 
 def serving_bench(rates, n_requests=128):
@@ -220,13 +252,17 @@ def serving_bench(rates, n_requests=128):
     out = {"groups": len(groups), "requests": n_requests,
            "input_bytes": int(sum(len(d) for _, _, d in base)), "rates": []}
     for rate in rates:
-        rng = np.random.default_rng(int(rate))
-        t = np.cumsum(rng.exponential(1.0 / rate, len(base)))
+        if np.isfinite(rate):
+            rng = np.random.default_rng(int(rate))
+            t = np.cumsum(rng.exponential(1.0 / rate, len(base)))
+        else:   # every request queued at once: the server's capacity
+            t = np.zeros(len(base))
         trace = [(float(t[i]), g, d) for i, (_, g, d) in enumerate(base)]
         res = run_poisson(srv, trace)
         exact = all(np.array_equal(res["reports"][i], alone[i]) for i in range(len(base)))
         row = {k: res[k] for k in ("p50_ms", "p99_ms", "mean_ms", "mbps", "makespan_s")}
-        row.update(rate_per_s=rate, bit_exact_vs_alone=exact)
+        row.update(rate_per_s=rate if np.isfinite(rate) else "all at once",
+                   bit_exact_vs_alone=exact)
         out["rates"].append(row)
         log(f"serving rate {rate}/s: {row}")
     srv.close()
@@ -253,23 +289,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="regex20k")
+    ap.add_argument("--workload", default="learned",
+                    choices=["learned", "batch1024", "regex20k", "literal64"])
     ap.add_argument("--depth", type=int, default=2)
-    ap.add_argument("--window", type=int, default=0,
-                    help="batched work-item positions, 0 = the engine's choice")
-    ap.add_argument("--cross-groups", type=int, default=1, choices=[0, 1, 2],
-                    help="batched static schedule: all-group work items (1 = when several groups)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned schedule")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
+    ap.add_argument("--also", default="batch1024",
+                    help="comma-separated further workloads compared the same way, or 'none'")
+    ap.add_argument("--serving", default="100,1000,10000,inf",
+                    help="arrival rates (per second, inf = all at once) for the serving run; "
+                         "'none' skips it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--only", choices=["plain", "batched"], default=None)
-    ap.add_argument("--learned", default="learned", choices=["none", "batch1024", "learned"],
-                    help="also run the plain / static / learned comparison on this workload")
-    ap.add_argument("--serving", default="100,1000,10000",
-                    help="arrival rates (per second) for the serving run; 'none' skips it")
+    ap.add_argument("--no-check", action="store_true", help="skip the oracle work counts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
-    ap.add_argument("--oracle-check", action="store_true",
-                    help="also compare reports with the oracle on three full chunks (slow)")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration << 20, max(1, a.steps))
@@ -279,94 +312,53 @@ def main():
                           "(the (group, chunk) units would shard across GPUs with no exchange)"}))
         sys.exit(2)
 
-    from ngap_amd import Engine
-    t = time.time()
-    groups, data, chunk = workload(a.workload)
-    total = (len(data) // chunk) * chunk
-    log(f"workload {a.workload}: {len(groups)} groups, {sum(g.num_states for g in groups)} states, "
-        f"{total} bytes, chunk {chunk} ({time.time() - t:.1f} s)")
-    eng = Engine(0)
-    t = time.time()
-    eng.load_groups(groups, max_precompute_depth=max(a.depth, 1))
-    eng.set_input(data, chunk)
-    log(f"loaded ({time.time() - t:.1f} s)")
-
-    common = dict(algorithm="nonblockingallgroups", precompute_depth=a.depth, unique=True)
-    cross = a.depth >= 2 and (a.cross_groups == 2 or (a.cross_groups == 1 and len(groups) > 1))
-    res = {}
-    for sched in (["plain", "batched"] if a.only is None else [a.only]):
-        opts = eng.options(schedule=sched, window=a.window, cross_groups=a.cross_groups, **common)
-        wall, kern, st = timed(eng, opts, a.steps, a.warmup)
-        res[sched] = dict(wall_ms=wall, kernel_ms=kern, stats=st, reports=eng.reports())
-        log(f"{sched}: {wall:.3f} ms/step wall, {kern:.3f} ms kernel, "
-            f"{total / wall / 1e3:.1f} MB/s, items {st['items']}, reports {res[sched]['reports'].size}")
-
-    main_s = "batched" if "batched" in res else "plain"
-    bit_exact = None
-    if not a.no_check and len(res) == 2:
-        bit_exact = bool(np.array_equal(res["plain"]["reports"], res["batched"]["reports"]))
-    oracle_check = None
-    per_byte = None
-    if not a.no_check:
-        per_byte, per_byte_ref, per_byte_cross, st, nbytes, dt, nunits = sample_stats(
-            groups, data, chunk, max(1, a.depth))
-        if main_s == "batched" and cross:
-            per_byte = per_byte_cross
-        log(f"oracle sample: {nbytes} bytes over {nunits} chunks in {dt:.1f} s, "
-            f"{per_byte:.2f} algorithmic bytes per input byte at memo depth {a.depth} "
-            f"({per_byte_ref:.1f} for the reference algorithm); {st}")
-        # reports on sampled chunks: engine == oracle restricted to those chunks
-        import oracle
-        S = len(data) // chunk
-        mask = np.zeros(len(groups) * S, np.uint8)
-        picks = [0, S // 2, S - 1]
-        for c in picks:
-            mask[c * len(groups):(c + 1) * len(groups)] = 1
-        if a.oracle_check or a.workload == "literal64":
-            want, _ = oracle.run(groups, data, chunk, S, unit_mask=mask)
-            got = res[main_s]["reports"]
-            off = (got & np.uint64(0xFFFFFFFF)).astype(np.int64)
-            keep = np.isin(off // chunk, picks)
-            oracle_check = bool(np.array_equal(got[keep], want))
-            log(f"oracle check on chunks {picks}: {oracle_check}")
-
-    wall = res[main_s]["wall_ms"]
-    kern = res[main_s]["kernel_ms"]
-    value = total / (wall * 1e-3) / 1e6
+    res, runs, groups, data, chunk = compare(a.workload, a.depth, a.steps, a.warmup,
+                                             learned_on=not a.no_learned, tune=not a.no_tune)
+    main_s = "learned" if "learned" in runs else "static"
+    total = res["input_bytes"]
+    wall = runs[main_s]["wall_ms"]
+    kern = runs[main_s]["kernel_ms"]
     roofline = None
-    if per_byte is not None:
-        achieved = per_byte * total / (kern * 1e-3) / 1e9
+    if not a.no_check:
+        depth = max(1, a.depth)
+        per_byte, per_byte_ref, per_byte_cross, st, nbytes, dt, nunits = sample_stats(
+            groups, data, chunk, depth)
+        cross = depth >= 2 and len(groups) > 1
+        model = per_byte_cross if cross else per_byte
+        log(f"oracle sample: {nbytes} bytes over {nunits} chunks in {dt:.1f} s, "
+            f"{model:.2f} algorithmic bytes per input byte ({per_byte:.1f} group by group, "
+            f"{per_byte_ref:.1f} for the reference algorithm); {st}")
+        achieved = model * total / (kern * 1e-3) / 1e9
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
-                        kernel="k_batched" if main_s == "batched" else "k_plain",
-                        work_items="all-group" if main_s == "batched" and cross else "per-group",
-                        algorithmic_bytes_per_input_byte=round(per_byte, 3),
+                        frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel="k_batched",
+                        work_items="all-group" if cross else "per-group",
+                        algorithmic_bytes_per_input_byte=round(model, 3),
+                        per_group_model_bytes_per_input_byte=round(per_byte, 3),
                         reference_model_bytes_per_input_byte=round(per_byte_ref, 2))
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "MB/s", "n_gpus": 1,
+        "metric": METRIC, "value": res[f"{main_s}_mbps"], "unit": "MB/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic (seeded generators, ngap_amd/synth.py)",
-        "config": {"workload": a.workload, "groups": len(groups),
-                   "states": int(sum(g.num_states for g in groups)), "input_bytes": int(total),
-                   "chunk": int(chunk), "algorithm": "nonblockingallgroups",
-                   "precompute_depth": a.depth, "window": a.window,
-                   "all_group_work_items": bool(cross)},
-        "kernel_ms": round(kern, 4), "kernel_mbps": round(total / (kern * 1e-3) / 1e6, 3),
-        "plain_mbps": round(total / (res["plain"]["wall_ms"] * 1e-3) / 1e6, 3) if "plain" in res else None,
-        "batched_mbps": round(total / (res["batched"]["wall_ms"] * 1e-3) / 1e6, 3) if "batched" in res else None,
-        "speedup_batched_vs_plain": round(res["plain"]["wall_ms"] / res["batched"]["wall_ms"], 3)
-        if len(res) == 2 else None,
-        "reports": int(res[main_s]["reports"].size),
-        "bit_exact_batched_vs_plain": bit_exact, "bit_exact_vs_oracle_sampled": oracle_check,
+        "config": {"workload": a.workload, "groups": res["groups"], "states": res["states"],
+                   "input_bytes": total, "chunk": res["chunk"],
+                   "algorithm": "nonblockingallgroups", "precompute_depth": a.depth,
+                   "engine": f"batched+{main_s}"},
+        "kernel_ms": round(kern, 4),
+        "plain_mbps": res["plain_mbps"],
+        "speedup_vs_plain": round(runs["plain"]["wall_ms"] / wall, 3),
+        "bit_exact_vs_plain": res["bit_exact_vs_plain"],
+        "engines": res,
         "roofline": roofline,
     }
-    eng.close()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(groups, data, chunk)
         log(f"cpu baseline: {out['cpu_baseline']}")
-    if a.learned != "none":
-        out["learned"] = learned_comparison(max(1, a.depth), a.steps, a.learned)
+    del runs, groups, data
+    if a.also != "none":
+        out["also"] = [compare(w, a.depth, a.steps, a.warmup, learned_on=not a.no_learned,
+                               tune=not a.no_tune)[0]
+                       for w in a.also.split(",") if w and w != a.workload]
     if a.serving != "none":
         out["serving"] = serving_bench([float(r) for r in a.serving.split(",")])
     print(json.dumps(out), flush=True)

@@ -58,6 +58,7 @@ struct KParams {
   uint64_t *spill;
   uint32_t spill_cap;      // items per wave
   uint32_t *overflow;      // plain: per unit; batched: per task
+  uint32_t *overflow_count;  // units / tasks that set their overflow flag
   unsigned long long *items;   // expanded items (statistic)
   const Task *tasks;
   const uint32_t *task_order;  // nullable

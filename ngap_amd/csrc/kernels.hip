@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_plain(KParams P) {
   drain<false>(P, W, src);
   flush_reports(P, W);
   if (W.lane == 0) {
-    if (W.overflow) atomicOr(&P.overflow[unit], 1u);
+    if (W.overflow && atomicOr(&P.overflow[unit], 1u) == 0u) atomicAdd(P.overflow_count, 1u);
     atomicAdd(P.items, W.items);
   }
 }
@@ -629,7 +629,10 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
       drain<false>(P, W, src);
     }
     if (W.lane == 0) {
-      if (W.overflow) P.overflow[ti] = 1u;
+      if (W.overflow) {
+        P.overflow[ti] = 1u;
+        atomicAdd(P.overflow_count, 1u);
+      }
       if (P.task_items) P.task_items[ti] = W.items - items0;
     }
   }
@@ -746,7 +749,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
   }
   flush_reports(P, W);
   if (W.lane == 0) {
-    if (W.overflow) atomicOr(&P.overflow[unit], 1u);
+    if (W.overflow && atomicOr(&P.overflow[unit], 1u) == 0u) atomicAdd(P.overflow_count, 1u);
     atomicAdd(P.items, W.items);
   }
 }
@@ -755,24 +758,26 @@ __global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
 // re-run of overflowed work items keeps the report count (keep_reports).
 __global__ void k_reset(unsigned long long *report_count, unsigned long long *items,
                         uint32_t *task_head, uint32_t *overflow, uint32_t n_overflow,
-                        int keep_reports) {
+                        uint32_t *overflow_count, int keep_reports) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     if (!keep_reports) *report_count = 0;
     *items = 0;
     *task_head = 0;
+    *overflow_count = 0;
   }
   for (; i < n_overflow; i += gridDim.x * blockDim.x) overflow[i] = 0;
 }
 
 // Host-side launch wrappers (this translation unit owns the kernels).
 hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
-                        uint32_t *head, uint32_t *ovf, uint32_t n, int keep_reports) {
+                        uint32_t *head, uint32_t *ovf, uint32_t n, uint32_t *ovf_count,
+                        int keep_reports) {
   uint32_t blocks = (n + 255) / 256;
   if (blocks == 0) blocks = 1;
   if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(256), 0, s, rc, items, head, ovf, n,
-                     keep_reports);
+                     ovf_count, keep_reports);
   return hipGetLastError();
 }
 

@@ -23,7 +23,8 @@ int ngap_fail(int code, const std::string &msg) {
 namespace ngapamd {
 
 hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
-                        uint32_t *head, uint32_t *ovf, uint32_t n, int keep_reports);
+                        uint32_t *head, uint32_t *ovf, uint32_t n, uint32_t *ovf_count,
+                        int keep_reports);
 hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
 hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
@@ -151,13 +152,17 @@ void attach_ctx(KParams &P, RunCtx &ctx) {
   P.report_count = counters + 0;
   P.items = counters + 1;
   P.task_head = reinterpret_cast<uint32_t *>(counters + 2);
+  P.overflow_count = reinterpret_cast<uint32_t *>(counters + 3);
   P.reports = ctx.reports.as<uint64_t>();
   P.report_cap = ctx.report_cap;
 }
 
 int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks) {
-  int per_cu = 0;
-  HIP_TRY(batched_occupancy(&per_cu, lds_bytes()));
+  int per_cu = eng->batched_per_cu;
+  if (per_cu == 0) {
+    HIP_TRY(batched_occupancy(&per_cu, lds_bytes()));
+    const_cast<ngap_engine *>(eng)->batched_per_cu = per_cu;
+  }
   if (per_cu < 1) per_cu = 1;
   *blocks = requested ? requested : (uint32_t)(per_cu * eng->num_cus);
   return NGAP_OK;
@@ -178,7 +183,7 @@ static int launch_batched_ctx(RunCtx &ctx, const KParams &P, uint32_t blocks, ui
       ctx.drop_graph();
       HIP_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
       hipError_t e1 = launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
-                                   n_flags, keep_reports);
+                                   n_flags, P.overflow_count, keep_reports);
       hipError_t e2 = launch_batched(ctx.stream, P, blocks, lds);
       hipGraph_t g = nullptr;
       hipError_t e3 = hipStreamEndCapture(ctx.stream, &g);
@@ -193,7 +198,7 @@ static int launch_batched_ctx(RunCtx &ctx, const KParams &P, uint32_t blocks, ui
     return NGAP_OK;
   }
   HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, n_flags,
-                       keep_reports));
+                       P.overflow_count, keep_reports));
   HIP_TRY(launch_batched(ctx.stream, P, blocks, lds));
   return NGAP_OK;
 }
@@ -202,7 +207,14 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
               const ngap_run_opts &o, ngap_run_stats *st) {
   if (tasks.size() >= (1u << 31)) return ngap_fail(NGAP_ERR_INVALID, "too many work items");
   const uint32_t T = (uint32_t)tasks.size();
-  HIP_TRY(ctx.tasks.upload(tasks));
+  // the work items stay on the device between runs with the same list
+  if (!ctx.tasks_resident || tasks.size() != ctx.last_tasks.size() ||
+      (T && memcmp(tasks.data(), ctx.last_tasks.data(), T * sizeof(Task)) != 0)) {
+    ctx.tasks_resident = false;
+    HIP_TRY(ctx.tasks.upload(tasks));
+    ctx.last_tasks = tasks;
+    ctx.tasks_resident = true;
+  }
   HIP_TRY(ctx.task_items.reserve((size_t)std::max<uint32_t>(T, 1) * 8));
   HIP_TRY(ctx.overflow.reserve((size_t)std::max<uint32_t>(T, 1) * 4));
   P.tasks = ctx.tasks.as<Task>();
@@ -225,20 +237,21 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   HIP_TRY(hipEventSynchronize(ctx.ev1));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
-  unsigned long long cnt[2] = {0, 0};
-  HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 16, hipMemcpyDeviceToHost));
-  std::vector<uint32_t> flags(T);
-  if (T) HIP_TRY(hipMemcpy(flags.data(), P.overflow, T * 4ull, hipMemcpyDeviceToHost));
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 32, hipMemcpyDeviceToHost));
   std::vector<uint32_t> bad;
-  for (uint32_t i = 0; i < T; i++)
-    if (flags[i]) bad.push_back(i);
+  if (cnt[3] && T) {  // per-item flags only when some work item overflowed
+    std::vector<uint32_t> flags(T);
+    HIP_TRY(hipMemcpy(flags.data(), P.overflow, T * 4ull, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < T; i++)
+      if (flags[i]) bad.push_back(i);
+  }
   if (st) {
     st->kernel_ms = ms;
     st->items = cnt[1];
     st->overflow_units = bad.size();
     st->num_tasks = T;
   }
-  ctx.last_tasks = tasks;
   // Overflow fallback: re-run only the overflowed work items, on a small grid
   // with a worklist 16x larger each round; their earlier partial reports stay
   // (the set union is exact).
@@ -363,12 +376,21 @@ std::vector<Task> cross_schedule(const ngap_engine *eng, uint32_t window, bool i
 }
 
 // Host work items -> device tasks, checking that each (group, chunk) unit is
-// tiled exactly once.
+// tiled exactly once (one sort over the work items, O(T log T)).
 int host_tasks(const ngap_engine *eng, const ngap_run_opts &o, int memo_depth,
                std::vector<Task> &out) {
   const uint32_t G = (uint32_t)eng->img.num_groups, S = eng->num_seg, L = (uint32_t)eng->chunk_len;
-  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per((size_t)G * S), all(S);
+  // (chunk, group or G for all groups, begin, end)
+  struct Range {
+    uint32_t chunk, group, begin, end;
+  };
+  std::vector<Range> rs;
+  rs.reserve(o.num_tasks);
   out.clear();
+  out.reserve(o.num_tasks);
+  auto bad_tiling = [] {
+    return ngap_fail(NGAP_ERR_INVALID, "work items must tile every (group, chunk) exactly once");
+  };
   for (uint32_t i = 0; i < o.num_tasks; i++) {
     const ngap_task &t = o.tasks[i];
     const bool every = t.group == NGAP_ALL_GROUPS;
@@ -376,32 +398,43 @@ int host_tasks(const ngap_engine *eng, const ngap_run_opts &o, int memo_depth,
       return ngap_fail(NGAP_ERR_INVALID, "work item " + std::to_string(i) + " out of range");
     if (every && memo_depth < 2)
       return ngap_fail(NGAP_ERR_INVALID, "all-group work items need precompute depth >= 2");
-    if (every) all[t.chunk].emplace_back(t.begin, t.end);
-    else per[(size_t)t.chunk * G + t.group].emplace_back(t.begin, t.end);
+    rs.push_back({t.chunk, every ? G : t.group, t.begin, t.end});
     const uint32_t cs = t.chunk * L;
     const uint32_t seed = (t.begin == 0 && (t.chunk == 0 || o.is_dup)) ? kTaskSeedStart : 0u;
-    out.push_back({t.group, cs, cs + L, cs + t.begin, cs + t.end, seed});
+    out.push_back({every ? kAllGroups : t.group, cs, cs + L, cs + t.begin, cs + t.end, seed});
   }
-  for (uint32_t c = 0; c < S; c++) {
-    if (all[c].empty()) continue;
-    for (uint32_t g = 0; g < G; g++)
-      if (!per[(size_t)c * G + g].empty())
+  std::sort(rs.begin(), rs.end(), [](const Range &x, const Range &y) {
+    if (x.chunk != y.chunk) return x.chunk < y.chunk;
+    if (x.group != y.group) return x.group < y.group;
+    return x.begin < y.begin;
+  });
+  // walk (chunk, group) runs: each must tile [0, L); a chunk covers all G
+  // groups either with per-group runs or with one all-group run
+  size_t i = 0;
+  uint64_t covered = 0;
+  while (i < rs.size()) {
+    const uint32_t c = rs[i].chunk, g = rs[i].group;
+    uint32_t at = 0;
+    for (; i < rs.size() && rs[i].chunk == c && rs[i].group == g; i++) {
+      if (rs[i].begin != at) return bad_tiling();
+      at = rs[i].end;
+    }
+    if (at != L) return bad_tiling();
+    covered += g == G ? G : 1;
+  }
+  // per-group and all-group runs in one chunk: detect by counting per chunk
+  {
+    size_t j = 0;
+    while (j < rs.size()) {
+      const uint32_t c = rs[j].chunk;
+      bool per = false, all = false;
+      for (; j < rs.size() && rs[j].chunk == c; j++) (rs[j].group == G ? all : per) = true;
+      if (per && all)
         return ngap_fail(NGAP_ERR_INVALID,
                          "a chunk with all-group work items takes no per-group work items");
-    per[(size_t)c * G].swap(all[c]);  // checked below like one group's tiling
-    for (uint32_t g = 1; g < G; g++) per[(size_t)c * G + g] = {{0u, L}};
-  }
-  for (auto &v : per) {
-    std::sort(v.begin(), v.end());
-    uint32_t at = 0;
-    for (auto &r : v) {
-      if (r.first != at)
-        return ngap_fail(NGAP_ERR_INVALID, "work items must tile every (group, chunk) exactly once");
-      at = r.second;
     }
-    if (at != L)
-      return ngap_fail(NGAP_ERR_INVALID, "work items must tile every (group, chunk) exactly once");
   }
+  if (covered != (uint64_t)G * S) return bad_tiling();
   return NGAP_OK;
 }
 
@@ -428,19 +461,17 @@ int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
   P.overflow = ctx.overflow.as<uint32_t>();
   attach_ctx(P, ctx);
   HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
-  HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, units, 0));
+  HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, units,
+                       P.overflow_count, 0));
   if (blocking) HIP_TRY(launch_blocking(ctx.stream, P, units, lds_bytes()));
   else HIP_TRY(launch_plain(ctx.stream, P, units, lds_bytes()));
   HIP_TRY(hipEventRecord(ctx.ev1, ctx.stream));
   HIP_TRY(hipEventSynchronize(ctx.ev1));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
-  std::vector<uint32_t> flags(units);
-  HIP_TRY(hipMemcpy(flags.data(), P.overflow, units * 4ull, hipMemcpyDeviceToHost));
-  uint64_t overflowed = 0;
-  for (uint32_t f : flags) overflowed += f != 0;
-  unsigned long long cnt[2] = {0, 0};
-  HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 16, hipMemcpyDeviceToHost));
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 32, hipMemcpyDeviceToHost));
+  const uint64_t overflowed = (uint32_t)cnt[3];
   ctx.last_count = cnt[0];
   ctx.last_report_off = o.report_off;
   ctx.have_run = true;

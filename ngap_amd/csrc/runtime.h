@@ -54,6 +54,7 @@ struct RunCtx {
   DevBuf reports, counters, overflow, spill, tasks, task_items, fb_tasks, fb_spill;
   uint64_t report_cap = 0, last_count = 0;
   bool last_report_off = false, have_run = false;
+  bool tasks_resident = false;  // `tasks` holds last_tasks
   std::vector<Task> last_tasks;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -78,6 +79,7 @@ uint32_t lds_bytes();
 struct ngap_engine {
   int device = 0;
   int num_cus = 0;
+  int batched_per_cu = 0;  // k_batched occupancy (blocks per CU), 0 = not queried
   ngapamd::Image img;
   bool loaded = false;
   ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;

@@ -18,7 +18,8 @@
 
 namespace ngapamd {
 hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
-                        uint32_t *head, uint32_t *ovf, uint32_t n, int keep_reports);
+                        uint32_t *head, uint32_t *ovf, uint32_t n, uint32_t *ovf_count,
+                        int keep_reports);
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
 }  // namespace ngapamd
 
@@ -110,6 +111,7 @@ int launch_slot(ngap_server *s, Slot &sl) {
   HIP_TRY(sl.input.reserve(total + 16));
   HIP_TRY(hipMemcpyAsync(sl.input.p, buf.data(), total, hipMemcpyHostToDevice, sl.ctx.stream));
   HIP_TRY(sl.ctx.tasks.upload(sl.tasks));
+  sl.ctx.tasks_resident = false;
   HIP_TRY(sl.ctx.overflow.reserve(sl.tasks.size() * 4 + 4));
   HIP_TRY(sl.ctx.reports.reserve(report_cap * 8));
   sl.ctx.report_cap = report_cap;
@@ -127,7 +129,7 @@ int launch_slot(ngap_server *s, Slot &sl) {
   sl.P = P;
   sl.blocks = s->blocks_per_launch;
   HIP_TRY(launch_reset(sl.ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
-                       P.num_tasks, 0));
+                       P.num_tasks, P.overflow_count, 0));
   HIP_TRY(launch_batched(sl.ctx.stream, P, sl.blocks, lds_bytes()));
   HIP_TRY(hipEventRecord(sl.ctx.ev1, sl.ctx.stream));
   sl.bytes = total + report_cap * 8 + spill_bytes;
@@ -138,13 +140,10 @@ int launch_slot(ngap_server *s, Slot &sl) {
 
 int retire_slot(ngap_server *s, Slot &sl) {
   HIP_TRY(hipEventSynchronize(sl.ctx.ev1));
-  const uint32_t T = sl.P.num_tasks;
-  std::vector<uint32_t> flags(T);
-  HIP_TRY(hipMemcpy(flags.data(), sl.P.overflow, T * 4ull, hipMemcpyDeviceToHost));
-  unsigned long long cnt = 0;
-  HIP_TRY(hipMemcpy(&cnt, sl.ctx.counters.p, 8, hipMemcpyDeviceToHost));
-  bool redo = cnt > sl.ctx.report_cap;
-  for (uint32_t f : flags) redo |= f != 0;
+  unsigned long long c4[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpy(c4, sl.ctx.counters.p, 32, hipMemcpyDeviceToHost));
+  const unsigned long long cnt = c4[0];
+  const bool redo = cnt > sl.ctx.report_cap || c4[3] != 0;
   if (redo) {
     // overflow: re-run the whole launch synchronously with room for every
     // report and the engine's overflow fallback for the worklists
