@@ -140,7 +140,8 @@ def learned_schedule(eng, groups, train, test, chunk, depth, waves=4096):
         keep = np.random.default_rng(0).choice(x.shape[0], 1000000, replace=False)
         x, y = x[keep], y[keep]
     m = MLP(0, seed=7)
-    loss = m.train(x, y, epochs=4, batch=4096, lr=2e-3)
+    epochs, batch = learned.train_settings(x.shape[0])
+    loss = m.train(x, y, epochs=epochs, batch=batch, lr=2e-3)
     t_train = time.time() - t0
     eng.set_input(test, chunk)
     t1 = time.time()

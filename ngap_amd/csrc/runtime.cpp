@@ -204,17 +204,22 @@ static int launch_batched_ctx(RunCtx &ctx, const KParams &P, uint32_t blocks, ui
 }
 
 int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> &tasks,
-              const ngap_run_opts &o, ngap_run_stats *st) {
+              uint64_t sched_id, const ngap_run_opts &o, ngap_run_stats *st) {
   if (tasks.size() >= (1u << 31)) return ngap_fail(NGAP_ERR_INVALID, "too many work items");
   const uint32_t T = (uint32_t)tasks.size();
-  // the work items stay on the device between runs with the same list
-  if (!ctx.tasks_resident || tasks.size() != ctx.last_tasks.size() ||
-      (T && memcmp(tasks.data(), ctx.last_tasks.data(), T * sizeof(Task)) != 0)) {
+  // the work items stay on the device between runs of the same schedule
+  const bool same = ctx.tasks_resident &&
+                    ((sched_id && sched_id == ctx.resident_sched) ||
+                     (tasks.size() == ctx.last_tasks.size() &&
+                      (T == 0 || memcmp(tasks.data(), ctx.last_tasks.data(),
+                                        T * sizeof(Task)) == 0)));
+  if (!same) {
     ctx.tasks_resident = false;
     HIP_TRY(ctx.tasks.upload(tasks));
     ctx.last_tasks = tasks;
     ctx.tasks_resident = true;
   }
+  ctx.resident_sched = sched_id;
   HIP_TRY(ctx.task_items.reserve((size_t)std::max<uint32_t>(T, 1) * 8));
   HIP_TRY(ctx.overflow.reserve((size_t)std::max<uint32_t>(T, 1) * 4));
   P.tasks = ctx.tasks.as<Task>();
@@ -289,7 +294,7 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
       if (f2[i]) still.push_back(bad[i]);
     bad.swap(still);
   }
-  HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 16, hipMemcpyDeviceToHost));
+  if (fb_tasks) HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 16, hipMemcpyDeviceToHost));
   ctx.last_count = cnt[0];
   ctx.last_report_off = o.report_off;
   ctx.have_run = true;
@@ -476,6 +481,8 @@ int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
   ctx.last_report_off = o.report_off;
   ctx.have_run = true;
   ctx.last_tasks.clear();
+  ctx.tasks_resident = false;
+  ctx.resident_sched = 0;
   if (st) {
     st->kernel_ms = ms;
     st->reports_raw = cnt[0];
@@ -581,6 +588,7 @@ int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t nu
     HIP_TRY(eng->d_sel[d].upload(im.memo[d].sel));
   }
   eng->loaded = true;
+  eng->generation++;
   eng->ctx.have_run = false;
   eng->ctx.drop_graph();
   return NGAP_OK;
@@ -604,6 +612,7 @@ int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
   eng->chunk_len = chunk_len;
   eng->num_seg = num_seg;
   eng->have_input = true;
+  eng->generation++;
   eng->ctx.have_run = false;
   eng->ctx.drop_graph();
   return NGAP_OK;
@@ -633,20 +642,48 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   if (st) st->num_units = (uint32_t)units;
   if (o.schedule == NGAP_SCHED_PLAIN || o.algorithm == NGAP_ALGO_BLOCKING)
     return run_plain(eng, ctx, P, o, st);
-  std::vector<Task> tasks;
-  if (o.tasks) {
-    int rc = host_tasks(eng, o, mp.memo_depth, tasks);
-    if (rc) return rc;
-  } else if (mp.memo_depth >= 2 &&
-             (o.cross_groups == 2 || (o.cross_groups == 1 && eng->img.num_groups > 1))) {
+  // the schedule: host work items, all-group windows or per-group windows,
+  // rebuilt only when its inputs change
+  ngap_engine::SchedCache &sc = eng->sched;
+  const int kind = o.tasks ? 0
+                   : (mp.memo_depth >= 2 && (o.cross_groups == 2 ||
+                                             (o.cross_groups == 1 && eng->img.num_groups > 1)))
+                       ? 1 : 2;
+  uint32_t waves = 0;
+  if (kind == 1) {
     uint32_t blocks = 0;
     int rc = persistent_blocks(eng, o.persistent_blocks, &blocks);
     if (rc) return rc;
-    tasks = cross_schedule(eng, o.window, o.is_dup != 0, blocks * P.waves_per_block);
-  } else {
-    tasks = static_schedule(eng, o.window, o.is_dup != 0);
+    waves = blocks * P.waves_per_block;
   }
-  return run_tasks(eng, ctx, P, tasks, o, st);
+  bool hit = sc.id && sc.generation == eng->generation && sc.kind == kind &&
+             sc.is_dup == (o.is_dup != 0);
+  if (hit && kind == 0)
+    hit = sc.memo_depth == mp.memo_depth && sc.host.size() == o.num_tasks &&
+          (o.num_tasks == 0 ||
+           memcmp(sc.host.data(), o.tasks, o.num_tasks * sizeof(ngap_task)) == 0);
+  else if (hit)
+    hit = sc.window == o.window && sc.waves == waves;
+  if (!hit) {
+    sc.id = 0;
+    if (kind == 0) {
+      int rc = host_tasks(eng, o, mp.memo_depth, sc.tasks);
+      if (rc) return rc;
+      sc.host.assign(o.tasks, o.tasks + o.num_tasks);
+    } else if (kind == 1) {
+      sc.tasks = cross_schedule(eng, o.window, o.is_dup != 0, waves);
+    } else {
+      sc.tasks = static_schedule(eng, o.window, o.is_dup != 0);
+    }
+    sc.generation = eng->generation;
+    sc.kind = kind;
+    sc.is_dup = o.is_dup != 0;
+    sc.memo_depth = mp.memo_depth;
+    sc.window = o.window;
+    sc.waves = waves;
+    sc.id = eng->next_sched_id++;
+  }
+  return run_tasks(eng, ctx, P, sc.tasks, sc.id, o, st);
 }
 
 int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,

@@ -55,6 +55,7 @@ struct RunCtx {
   uint64_t report_cap = 0, last_count = 0;
   bool last_report_off = false, have_run = false;
   bool tasks_resident = false;  // `tasks` holds last_tasks
+  uint64_t resident_sched = 0;  // schedule id of last_tasks (0 = unknown)
   std::vector<Task> last_tasks;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -92,6 +93,18 @@ struct ngap_engine {
   bool have_input = false;
   double byte_freq[256] = {0};  // input byte histogram (normalised)
   ngapamd::RunCtx ctx;          // the context ngap_run uses
+  // last schedule built by ngap_run, reused while its inputs are unchanged
+  struct SchedCache {
+    uint64_t id = 0;            // 0 = empty; ids are never reused
+    uint64_t generation = 0;    // engine generation it was built for
+    int kind = -1;              // 0 host work items, 1 all-group, 2 per-group
+    uint32_t window = 0, waves = 0;
+    int is_dup = 0, memo_depth = 0;
+    std::vector<ngap_task> host;
+    std::vector<ngapamd::Task> tasks;
+  } sched;
+  uint64_t generation = 1;      // bumped by ngap_load_groups / ngap_set_input
+  uint64_t next_sched_id = 1;
 };
 
 namespace ngapamd {
@@ -100,6 +113,7 @@ void attach_ctx(KParams &P, RunCtx &ctx);
 int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks);
 // Runs tasks on ctx (batched kernel), with the overflow fallback of o.
 int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> &tasks,
+              uint64_t sched_id,
               const ngap_run_opts &o, ngap_run_stats *st);
 // Device reports of ctx's last run, sorted and unique (device encoding).
 int sorted_reports(RunCtx &ctx, std::vector<uint64_t> &out);

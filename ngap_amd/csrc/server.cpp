@@ -154,7 +154,7 @@ int retire_slot(ngap_server *s, Slot &sl) {
       HIP_TRY(sl.ctx.reports.reserve(cnt * 8 + 8));
       sl.ctx.report_cap = cnt + 1;
     }
-    int rc = run_tasks(s->eng, sl.ctx, P, sl.tasks, o, nullptr);
+    int rc = run_tasks(s->eng, sl.ctx, P, sl.tasks, 0, o, nullptr);
     if (rc) {
       for (uint64_t id : sl.ids) s->reqs[id].state = 3;
       sl.busy = false;

@@ -86,6 +86,14 @@ def collect_traces(engine, data, chunk, num_groups, depth=2, window=WINDOW):
     return pairs_of(done, chunk, window), np.log1p(items).astype(np.float32)
 
 
+def train_settings(n, steps=2000):
+    """(epochs, batch) giving about `steps` Adam steps (at least 4 epochs)
+    for n training samples."""
+    batch = int(min(4096, max(256, n // 16)))
+    per_epoch = max(1, n // batch)
+    return int(min(300, max(4, -(-steps // per_epoch)))), batch
+
+
 def features(hist, stats, pairs):
     pairs = np.asarray(pairs, np.int64)
     return np.concatenate([hist[pairs[:, 1]], stats[pairs[:, 0]]], 1).astype(np.float32)

@@ -108,6 +108,27 @@ def test_all_group_work_items(engine):
                    tasks=tasks + [(0, 0, 0, L)])
 
 
+def test_schedule_cache_follows_input_and_groups(engine):
+    groups, data = _workload(seed=25)
+    L = 1 << 13
+    engine.load_groups(groups, 2)
+    for k, d in enumerate([data, data[::-1].copy(), data]):
+        want, _ = oracle.run(groups, d, L)
+        engine.set_input(d, L)
+        for x in (0, 1):
+            say("schedule cache, input", k, "cross", x)
+            engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
+                       cross_groups=x)
+            assert np.array_equal(engine.reports(), want)
+    g2 = groups[:5]
+    want, _ = oracle.run(g2, data, L)
+    engine.load_groups(g2, 2)
+    engine.set_input(data, L)
+    say("schedule cache, fewer groups")
+    engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2)
+    assert np.array_equal(engine.reports(), want)
+
+
 def test_autotuner_picks_knobs_not_results(engine):
     groups, data = _workload(seed=24)
     L = 1 << 13
