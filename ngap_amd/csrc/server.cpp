@@ -10,7 +10,9 @@
 // There is no counterpart in the reference (it runs one input per process).
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <unordered_map>
 
@@ -32,9 +34,42 @@ double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Pinned host buffers for request bytes, reused by size class (a power of
+// two): the host-to-device copy of a launch is an asynchronous DMA from them.
+struct PinnedPool {
+  std::multimap<size_t, uint8_t *> free_;
+  std::vector<uint8_t *> all;
+  static size_t cls(size_t n) {
+    size_t c = 1 << 16;
+    while (c < n) c <<= 1;
+    return c;
+  }
+  uint8_t *get(size_t n, size_t *cap) {
+    const size_t c = cls(n);
+    *cap = c;
+    auto it = free_.find(c);
+    if (it != free_.end()) {
+      uint8_t *p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) return nullptr;
+    all.push_back((uint8_t *)p);
+    return (uint8_t *)p;
+  }
+  void put(uint8_t *p, size_t cap) {
+    if (p) free_.emplace(cap, p);
+  }
+  ~PinnedPool() {
+    for (uint8_t *p : all) (void)hipHostFree(p);
+  }
+};
+
 struct Request {
   int32_t group = 0;
-  std::vector<uint8_t> data;
+  uint8_t *data = nullptr;      // pinned, `cap` bytes, `len` used
+  uint64_t len = 0, cap = 0;
   double t_submit = 0, t_done = 0;
   int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed
   std::vector<uint64_t> reports;
@@ -43,7 +78,6 @@ struct Request {
 struct Slot {
   RunCtx ctx;
   DevBuf input;
-  std::vector<uint8_t> host;    // staging copy, alive until the launch retires
   bool busy = false;
   std::vector<uint64_t> ids;
   std::vector<uint32_t> base;   // request start offsets in the slot input
@@ -54,6 +88,8 @@ struct Slot {
 };
 
 constexpr uint32_t kServeWindow = 4096;
+constexpr uint32_t kServeSpill = 16384;  // worklist spill items per wave
+constexpr uint64_t kMinLaunchBytes = 1 << 20;
 
 }  // namespace
 
@@ -67,7 +103,9 @@ struct ngap_server {
   std::unordered_map<uint64_t, Request> reqs;
   uint64_t next_id = 1;
   uint64_t inflight_bytes = 0;
+  uint64_t queued_bytes = 0;
   uint32_t blocks_per_launch = 0;
+  PinnedPool pinned;  // freed by ngap_server_destroy after every launch completed
 };
 
 namespace {
@@ -78,25 +116,33 @@ int launch_slot(ngap_server *s, Slot &sl) {
   sl.base.clear();
   sl.tasks.clear();
   uint64_t total = 0;
-  const uint64_t spill_bytes = (uint64_t)s->blocks_per_launch * 4 * 65536 * 8;
+  const uint64_t spill_bytes = (uint64_t)s->blocks_per_launch * 4 * kServeSpill * 8;
+  // continuous batching: a launch takes its share of what is queued (split
+  // over the idle slots), so later arrivals find a slot soon
+  uint32_t idle = 0;
+  for (auto &x : s->slots) idle += !x->busy;
+  const uint64_t share = std::max<uint64_t>(kMinLaunchBytes, s->queued_bytes / std::max(1u, idle));
   while (!s->queue.empty()) {
     const uint64_t id = s->queue.front();
     const Request &r = s->reqs[id];
-    const uint64_t need = total + r.data.size();
+    const uint64_t need = total + r.len;
     const uint64_t hbm = need + need * 16 + spill_bytes;  // input + reports + spill
-    if (!sl.ids.empty() && (s->inflight_bytes + hbm > s->budget || need >= (1ull << 31))) break;
+    if (!sl.ids.empty() &&
+        (s->inflight_bytes + hbm > s->budget || need >= (1ull << 31) || total >= share))
+      break;
     sl.ids.push_back(id);
     sl.base.push_back((uint32_t)total);
     total = need;
     s->queue.pop_front();
+    s->queued_bytes -= r.len;
   }
   if (sl.ids.empty()) return NGAP_OK;
-  std::vector<uint8_t> &buf = sl.host;
-  buf.resize(total);
+  HIP_TRY(sl.input.reserve(total + 16));
   for (size_t i = 0; i < sl.ids.size(); i++) {
     Request &r = s->reqs[sl.ids[i]];
-    std::copy(r.data.begin(), r.data.end(), buf.begin() + sl.base[i]);
-    const uint32_t cs = sl.base[i], ce = cs + (uint32_t)r.data.size();
+    HIP_TRY(hipMemcpyAsync(sl.input.as<uint8_t>() + sl.base[i], r.data, r.len,
+                           hipMemcpyHostToDevice, sl.ctx.stream));
+    const uint32_t cs = sl.base[i], ce = cs + (uint32_t)r.len;
     for (uint32_t a = cs; a < ce; a += kServeWindow) {
       const uint32_t b = std::min<uint32_t>(a + kServeWindow, ce);
       sl.tasks.push_back({(uint32_t)r.group, cs, ce, a, b, a == cs ? kTaskSeedStart : 0u});
@@ -108,8 +154,6 @@ int launch_slot(ngap_server *s, Slot &sl) {
     return (x.cend - x.cstart) > (y.cend - y.cstart);
   });
   const uint64_t report_cap = std::max<uint64_t>(1 << 20, total * 2);
-  HIP_TRY(sl.input.reserve(total + 16));
-  HIP_TRY(hipMemcpyAsync(sl.input.p, buf.data(), total, hipMemcpyHostToDevice, sl.ctx.stream));
   HIP_TRY(sl.ctx.tasks.upload(sl.tasks));
   sl.ctx.tasks_resident = false;
   HIP_TRY(sl.ctx.overflow.reserve(sl.tasks.size() * 4 + 4));
@@ -124,7 +168,7 @@ int launch_slot(ngap_server *s, Slot &sl) {
   P.num_tasks = (uint32_t)sl.tasks.size();
   P.overflow = sl.ctx.overflow.as<uint32_t>();
   P.spill = sl.ctx.spill.as<uint64_t>();
-  P.spill_cap = 65536;
+  P.spill_cap = kServeSpill;
   attach_ctx(P, sl.ctx);
   sl.P = P;
   sl.blocks = s->blocks_per_launch;
@@ -156,7 +200,12 @@ int retire_slot(ngap_server *s, Slot &sl) {
     }
     int rc = run_tasks(s->eng, sl.ctx, P, sl.tasks, 0, o, nullptr);
     if (rc) {
-      for (uint64_t id : sl.ids) s->reqs[id].state = 3;
+      for (uint64_t id : sl.ids) {
+        Request &r = s->reqs[id];
+        r.state = 3;
+        s->pinned.put(r.data, r.cap);
+        r.data = nullptr;
+      }
       sl.busy = false;
       s->inflight_bytes -= sl.bytes;
       return rc;
@@ -177,7 +226,7 @@ int retire_slot(ngap_server *s, Slot &sl) {
   for (size_t oi = 0; oi < order.size(); oi++) {
     const size_t i = order[oi];
     Request &r = s->reqs[sl.ids[i]];
-    const uint64_t lo = sl.base[i], hi = lo + r.data.size();
+    const uint64_t lo = sl.base[i], hi = lo + r.len;
     r.reports.clear();
     while (k < v.size() && (v[k] >> 32) < hi) {
       const uint64_t off = (v[k] >> 32) - lo;
@@ -186,7 +235,8 @@ int retire_slot(ngap_server *s, Slot &sl) {
     }
     r.state = 2;
     r.t_done = t;
-    std::vector<uint8_t>().swap(r.data);
+    s->pinned.put(r.data, r.cap);
+    r.data = nullptr;
   }
   sl.busy = false;
   s->inflight_bytes -= sl.bytes;
@@ -214,7 +264,9 @@ int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
   uint32_t blocks = 0;
   int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks);
   if (rc) return rc;
-  s->blocks_per_launch = std::max<uint32_t>(1, blocks / (uint32_t)max_inflight);
+  // every launch gets the whole persistent grid: launches in flight share the
+  // CUs, and a lone launch is not held to a fraction of the GPU
+  s->blocks_per_launch = std::max<uint32_t>(1, blocks);
   for (int i = 0; i < max_inflight; i++) {
     s->slots.emplace_back(new Slot());
     rc = s->slots.back()->ctx.init();
@@ -239,11 +291,18 @@ int ngap_server_submit(ngap_server *s, int32_t group, const uint8_t *data, uint6
     return ngap_fail(NGAP_ERR_INVALID, "group out of range");
   if (len == 0 || len >= (1ull << 31)) return ngap_fail(NGAP_ERR_INVALID, "bad request length");
   const uint64_t id = s->next_id++;
+  uint64_t cap = 0;
+  uint8_t *buf = s->pinned.get(len, &cap);
+  if (!buf) return ngap_fail(NGAP_ERR_OOM, "pinned host allocation failed");
+  memcpy(buf, data, len);
   Request &r = s->reqs[id];
   r.group = group;
-  r.data.assign(data, data + len);
+  r.data = buf;
+  r.len = len;
+  r.cap = cap;
   r.t_submit = now_ms();
   s->queue.push_back(id);
+  s->queued_bytes += len;
   *req_id = id;
   return NGAP_OK;
 }
