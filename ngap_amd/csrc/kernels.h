@@ -22,7 +22,7 @@ struct DevImage {
   const uint32_t *nrep[kMaxMemoDepth];
   const int32_t *list[kMaxMemoDepth];
   const uint32_t *sel[kMaxMemoDepth];  // rank tables of depths 2 and 3
-  const uint32_t *mask1;     // 8 words per group
+  const uint32_t *mask1;     // 16 words per group (layout.h)
   // cross-group root index (layout.h)
   const uint32_t *xr1, *xe1, *xmask1, *xr2, *xe2, *xsod;
   uint32_t n_xsod;
@@ -73,7 +73,7 @@ constexpr uint32_t kBlockThreads = 256;
 
 // LDS bytes per wave for the given capacities
 NGAP_HD inline uint32_t wave_lds_bytes(uint32_t ring, uint32_t rbuf, uint32_t dc) {
-  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64 + 64 + 8) * 4 + 4;
+  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64 + 64 + 16) * 4 + 4;
 }
 
 }  // namespace ngapamd

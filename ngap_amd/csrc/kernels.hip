@@ -66,7 +66,7 @@ struct WaveState {
   uint32_t *sh_beg;   // 64
   uint32_t *sh_pos;   // 64
   uint32_t *sh_tag;   // 64
-  uint32_t *mask1;    // 8: bytes that start a root in the current group(s)
+  uint32_t *mask1;    // 16: root bytes, then reporting depth-1 bytes (layout.h)
   uint64_t *spill;
   uint32_t head, tail, rn, spill_n;
   uint32_t lane;
@@ -203,65 +203,66 @@ NGAP_DEV int32_t memo_rank(const uint32_t *sel, size_t at, uint32_t bit) {
 // Roots for positions [p0, pend): one lane per position.  A position whose
 // byte starts no root costs one load and one bit test.  A lane uses the
 // deepest memo level that fits before the chunk end (kernel_ngap_OA.cu:243-
-// 244); shallower levels contribute their reports only.
+// 244); shallower levels contribute their reports only.  With depth >= 2 the
+// independent loads of all levels are issued together (input bytes, then the
+// rank pairs, then the list ranges) and the depth-1 lists are read only for
+// bytes whose list reports.
 NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
   const uint8_t *in = P.input;
   const uint32_t p = p0 + W.lane;
   const bool act = p < pend;
-  const uint32_t b0 = act ? in[p] : 0u;
-  bool has = act && ((W.mask1[b0 >> 5] >> (b0 & 31)) & 1u);
-  if (wballot(has) == 0) return;
   const uint32_t left = act ? W.cend - p : 0u;
   const uint32_t k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
+  const uint32_t b0 = act ? in[p] : 0u;
+  const uint32_t b1 = k >= 2 ? in[p + 1] : 0u;
+  const uint32_t b2 = k >= 3 ? in[p + 2] : 0u;
+  const bool has = act && ((W.mask1[b0 >> 5] >> (b0 & 31)) & 1u);
+  if (wballot(has) == 0) return;
   const uint32_t kReportOnly = 1u << 31;
-  // depth 1: all-input states matching b0
-  {
+  if (P.memo_depth < 2) {  // depth 1: all-input states matching b0 become items
     uint32_t beg = 0, cnt = 0;
     if (has) {
       const uint32_t li = W.g * 256u + b0;
       beg = P.img.beg[0][li];
-      cnt = k == 1 ? P.img.beg[0][li + 1] - beg : P.img.nrep[0][li];
-      if (k != 1) beg |= kReportOnly;
+      cnt = P.img.beg[0][li + 1] - beg;
+    }
+    flatten(P, W, cnt, beg, p, W.gtag, P.img.list[0]);
+    return;
+  }
+  // rank pair of the 2-gram (and the depth-1 reports of rare bytes)
+  int32_t li2 = -1;
+  if (has && k >= 2) {
+    const uint32_t key = (b0 << 8) | b1;
+    li2 = memo_rank(P.img.sel[1], (size_t)W.g * 2048 + (key >> 5), key & 31);
+  }
+  const bool rep1 = has && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u);
+  if (wballot(rep1)) {  // depth 1: reporting entries only (the last position included)
+    uint32_t beg = 0, cnt = 0;
+    if (rep1) {
+      const uint32_t li = W.g * 256u + b0;
+      beg = P.img.beg[0][li] | kReportOnly;
+      cnt = P.img.nrep[0][li];
     }
     flatten(P, W, cnt, beg, p, W.gtag, P.img.list[0]);
   }
-  if (P.memo_depth < 2) return;
-  // depth 2: the group's 2-gram rank table
-  bool has2 = has && k >= 2;
-  int32_t li2 = -1;
-  if (has2) {
-    const uint32_t key = (b0 << 8) | in[p + 1];
-    li2 = memo_rank(P.img.sel[1], (size_t)W.g * 2048 + (key >> 5), key & 31);
-    has2 = li2 >= 0;
-  }
+  const bool has2 = li2 >= 0;
   if (wballot(has2) == 0) return;
-  {
-    uint32_t beg = 0, cnt = 0;
-    if (has2) {
-      beg = P.img.beg[1][li2];
-      cnt = k == 2 ? P.img.beg[1][li2 + 1] - beg : P.img.nrep[1][li2];
-      if (k != 2) beg |= kReportOnly;
-    }
-    flatten(P, W, cnt, beg, p + 1, W.gtag, P.img.list[1]);
-  }
-  if (P.memo_depth < 3) return;
-  // depth 3: the rank table below the first two bytes' depth-2 key
-  bool has3 = has2 && k >= 3;
+  // depth-2 list range and the depth-3 rank pair
+  uint32_t beg2 = 0, cnt2 = 0;
   int32_t li3 = -1;
-  if (has3) {
-    const uint32_t b2 = in[p + 2];
-    li3 = memo_rank(P.img.sel[2], (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
-    has3 = li3 >= 0;
+  if (has2) {
+    beg2 = P.img.beg[1][li2];
+    cnt2 = k == 2 ? P.img.beg[1][li2 + 1] - beg2 : P.img.nrep[1][li2];
+    if (k != 2) beg2 |= kReportOnly;
+    if (k >= 3) li3 = memo_rank(P.img.sel[2], (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
   }
-  if (wballot(has3) == 0) return;
-  {
-    uint32_t beg = 0, cnt = 0;
-    if (has3) {
-      beg = P.img.beg[2][li3];
-      cnt = P.img.beg[2][li3 + 1] - beg;
-    }
-    flatten(P, W, cnt, beg, p + 2, W.gtag, P.img.list[2]);
+  uint32_t beg3 = 0, cnt3 = 0;
+  if (li3 >= 0) {
+    beg3 = P.img.beg[2][li3];
+    cnt3 = P.img.beg[2][li3 + 1] - beg3;
   }
+  flatten(P, W, cnt2, beg2, p + 1, W.gtag, P.img.list[1]);
+  if (wballot(cnt3 != 0)) flatten(P, W, cnt3, beg3, p + 2, W.gtag, P.img.list[2]);
 }
 
 // Distributes per-lane index ranges [e0, e0 + ecnt) over the wave, 64
@@ -298,18 +299,26 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
   const uint8_t *in = P.input;
   const uint32_t p = p0 + W.lane;
   const bool act = p < pend;
-  const uint32_t b0 = act ? in[p] : 0u;
   const uint32_t left = act ? W.cend - p : 0u;
   const uint32_t k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
+  const uint32_t b0 = act ? in[p] : 0u;
+  const uint32_t b1 = k >= 2 ? in[p + 1] : 0u;
+  const uint32_t b2 = k >= 3 ? in[p + 2] : 0u;
   const uint32_t kReportOnly = 1u << 31;
+  uint32_t e0 = 0, ec = 0;
+  if (k >= 2) {
+    const uint32_t key = (b0 << 8) | b1;
+    e0 = P.img.xr2[key];
+    ec = P.img.xr2[key + 1] - e0;
+  }
   {  // depth 1: reporting entries only (positions that continue use depth >= 2)
-    uint32_t e0 = 0, ec = 0;
-    if (act && ((W.mask1[b0 >> 5] >> (b0 & 31)) & 1u)) {
-      e0 = P.img.xr1[b0];
-      ec = P.img.xr1[b0 + 1] - e0;
+    uint32_t r0 = 0, rc = 0;
+    if (act && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u)) {
+      r0 = P.img.xr1[b0];
+      rc = P.img.xr1[b0 + 1] - r0;
     }
-    if (wballot(ec != 0)) {
-      for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
+    if (wballot(rc != 0)) {
+      for_entries(W, r0, rc, [&](bool ok, uint32_t idx, uint32_t o) {
         const uint32_t pos = __shfl(p, (int)o, 64);
         uint32_t beg = 0, cnt = 0, tag = 0;
         if (ok) {
@@ -322,38 +331,29 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
       });
     }
   }
-  uint32_t e0 = 0, ec = 0;
-  if (act && k >= 2) {
-    const uint32_t key = (b0 << 8) | in[p + 1];
-    e0 = P.img.xr2[key];
-    ec = P.img.xr2[key + 1] - e0;
-  }
   if (wballot(ec != 0) == 0) return;
   for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
     const uint32_t pos = __shfl(p, (int)o, 64);
     const uint32_t ko = __shfl(k, (int)o, 64);
-    uint32_t beg = 0, cnt = 0, tag = 0, li2 = 0;
+    const uint32_t c2 = __shfl(b2, (int)o, 64);
+    uint32_t beg = 0, cnt = 0, tag = 0;
+    int32_t li3 = -1;
     if (ok) {
-      const uint2 e = reinterpret_cast<const uint2 *>(P.img.xe2)[idx];
-      li2 = e.x;
-      tag = e.y << 22;
-      beg = P.img.beg[1][li2];
-      cnt = ko == 2 ? P.img.beg[1][li2 + 1] - beg : P.img.nrep[1][li2];
+      // (list index, begin, end, reporting entries | group << 22)
+      const uint4 e = reinterpret_cast<const uint4 *>(P.img.xe2)[idx];
+      tag = (e.w >> 22) << 22;
+      beg = e.y;
+      cnt = ko == 2 ? e.z - e.y : (e.w & (uint32_t)kIdMask);
       if (ko != 2) beg |= kReportOnly;
+      if (ko >= 3) li3 = memo_rank(P.img.sel[2], (size_t)e.x * 8 + (c2 >> 5), c2 & 31);
+    }
+    uint32_t beg3 = 0, cnt3 = 0;
+    if (li3 >= 0) {
+      beg3 = P.img.beg[2][li3];
+      cnt3 = P.img.beg[2][li3 + 1] - beg3;
     }
     flatten(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
-    if (P.memo_depth >= 3) {
-      uint32_t beg3 = 0, cnt3 = 0;
-      if (ok && ko >= 3) {
-        const uint32_t b2 = in[pos + 2];
-        const int32_t li3 = memo_rank(P.img.sel[2], (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
-        if (li3 >= 0) {
-          beg3 = P.img.beg[2][li3];
-          cnt3 = P.img.beg[2][li3 + 1] - beg3;
-        }
-      }
-      flatten(P, W, cnt3, beg3, pos + 2, tag, P.img.list[2]);
-    }
+    if (wballot(cnt3 != 0)) flatten(P, W, cnt3, beg3, pos + 2, tag, P.img.list[2]);
   });
 }
 
@@ -420,15 +420,26 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
   const bool can_priv = P.priv_limit < 0 || depth < P.priv_limit;
   while (wballot(more)) {
     int32_t link = -1;
-    int32_t es[4] = {rec.x, rec.y, rec.z, rec.w};
+    const int32_t es[4] = {rec.x, rec.y, rec.z, rec.w};
+    // all loads of this record first: the successors' class words and the
+    // continuation record, then the wave-wide emits and pushes
+    uint32_t cw[4];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int32_t e = es[s];
-      const bool edge = more && e >= 0;
       if (more && e <= -2) link = -(e + 2);
+      cw[s] = (more && e >= 0)
+                  ? P.img.cls[(size_t)(lg.cls_base + ((uint32_t)e & (uint32_t)kIdMask)) * 8 +
+                              (b >> 5)]
+                  : 0u;
+    }
+    int4 next = make_int4(-1, -1, -1, -1);
+    if (more && link >= 0) next = reinterpret_cast<const int4 *>(P.img.succ4)[link];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int32_t e = es[s];
       const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-      bool hit = false;
-      if (edge) hit = class_has(P, lg.cls_base, u, b);
+      const bool hit = (cw[s] >> (b & 31)) & 1u;
       const uint64_t it = item_of(u | lg.gtag, p1);
       emit(P, W, hit && (e & kFlagReport), it);
       bool keep = hit && !(e & kFlagDeadEnd) && (p1 + 1 < W.cend);
@@ -441,7 +452,7 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
       push(P, W, keep && !take, it);
     }
     more = more && link >= 0;
-    if (more) rec = reinterpret_cast<const int4 *>(P.img.succ4)[link];
+    rec = next;
   }
   if (has_next) {
     v = nv;
@@ -462,7 +473,7 @@ NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t cs
   W.cstart = cstart;
   W.cend = cend;
   for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
-  if (W.lane < 8) W.mask1[W.lane] = P.img.mask1[g * 8 + W.lane];
+  if (W.lane < 16) W.mask1[W.lane] = P.img.mask1[g * 16 + W.lane];
   wsync();
 }
 
@@ -473,7 +484,7 @@ NGAP_DEV void begin_cross(const KParams &P, WaveState &W, uint32_t cstart, uint3
   W.cstart = cstart;
   W.cend = cend;
   for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
-  if (W.lane < 8) W.mask1[W.lane] = P.img.xmask1[W.lane];
+  if (W.lane < 8) W.mask1[8 + W.lane] = P.img.xmask1[W.lane];  // bytes with depth-1 reports
   wsync();
 }
 

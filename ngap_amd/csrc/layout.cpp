@@ -156,12 +156,13 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
       for (int b = 0; b < 256; b++)
         if (has_byte(hg.sets, hg.aas[i], b)) l1[b].push_back(hg.aas[i]);
     size_t m1 = img.mask1.size();
-    img.mask1.resize(m1 + 8, 0u);
+    img.mask1.resize(m1 + 16, 0u);
     for (int b = 0; b < 256; b++) {
       std::sort(l1[b].begin(), l1[b].end());
       l1[b].erase(std::unique(l1[b].begin(), l1[b].end()), l1[b].end());
       if (!l1[b].empty()) img.mask1[m1 + b / 32] |= 1u << (b % 32);
       push_list(img.memo[0], l1[b], tag);
+      if (img.memo[0].nrep.back()) img.mask1[m1 + 8 + b / 32] |= 1u << (b % 32);
     }
     // deeper levels, each list's successors split by the next byte
     std::vector<char> mark(n, 0);
@@ -233,12 +234,15 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     img.xr2.assign(65537, 0);
     for (uint32_t k : key2) img.xr2[k + 1]++;
     for (int k = 0; k < 65536; k++) img.xr2[k + 1] += img.xr2[k];
-    img.xe2.resize((size_t)img.xr2[65536] * 2);
+    img.xe2.resize((size_t)img.xr2[65536] * 4);
     std::vector<uint32_t> at(img.xr2.begin(), img.xr2.end() - 1);
+    const MemoLevel &m2 = img.memo[1];
     for (size_t i = 0; i < key2.size(); i++) {  // list index i (depth-2 lists in order)
-      const uint32_t slot = at[key2[i]]++;
-      img.xe2[2 * (size_t)slot] = (uint32_t)i;
-      img.xe2[2 * (size_t)slot + 1] = grp2[i];
+      uint32_t *e = &img.xe2[4 * (size_t)at[key2[i]]++];
+      e[0] = (uint32_t)i;
+      e[1] = m2.beg[i];
+      e[2] = m2.beg[i + 1];
+      e[3] = m2.nrep[i] | (grp2[i] << 22);  // nrep < 2^22: at most the group's states
     }
   }
   for (auto &m : img.memo)

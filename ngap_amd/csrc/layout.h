@@ -87,14 +87,16 @@ struct Image {
   std::vector<uint32_t> cls;    // 8 per state
   std::vector<int32_t> sod;     // tagged SOD ids
   MemoLevel memo[kMaxMemoDepth];// memo[0] = depth 1 (always built)
-  // root filter: per group 8 words, bit b set when byte b has a depth-1 list
+  // root filters: per group 16 words; bit b of words 0-7 is set when byte b
+  // has a depth-1 list, of words 8-15 when that list has reporting entries
   std::vector<uint32_t> mask1;
   // Cross-group root index (memo depth >= 2), for work items that scan one
   // window for every group at once:
   //   x1: byte b -> depth-1 lists with reporting entries, xe1[xr1[b]..xr1[b+1])
   //       holding list indices g*256 + b; xmask1 marks bytes with any;
-  //   x2: 2-gram k -> depth-2 lists of every group, xe2 pairs (list index,
-  //       group) in xe2[2*xr2[k] .. 2*xr2[k+1]);
+  //   x2: 2-gram k -> depth-2 lists of every group: entries xr2[k] ..
+  //       xr2[k+1]-1 of xe2, each 4 words (list index, list begin, list end,
+  //       reporting entries | group << 22), one 16-byte load per entry;
   //   xsod: (tagged id, group) pairs of every group's start-of-data states.
   std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod;
   // statistics (for the learned scheduler's features and reports)

@@ -335,7 +335,7 @@ std::vector<Task> static_schedule(const ngap_engine *eng, uint32_t window, bool 
   std::vector<double> root_p(G, 0.0);
   for (uint32_t g = 0; g < G; g++)
     for (int b = 0; b < 256; b++)
-      if ((im.mask1[g * 8 + b / 32] >> (b % 32)) & 1u) root_p[g] += eng->byte_freq[b];
+      if ((im.mask1[g * 16 + b / 32] >> (b % 32)) & 1u) root_p[g] += eng->byte_freq[b];
   std::vector<Task> tasks;
   std::vector<double> cost;
   for (uint32_t c = 0; c < S; c++)

@@ -130,9 +130,12 @@ int main() {
     std::vector<std::vector<int>> x2(65536 * (size_t)ng, std::vector<int>());
     for (uint32_t k = 0; k < 65536; k++)
       for (uint32_t i = im.xr2[k]; i < im.xr2[k + 1]; i++) {
-        const uint32_t li = im.xe2[2 * i], g = im.xe2[2 * i + 1];
+        const uint32_t li = im.xe2[4 * i], g = im.xe2[4 * i + 3] >> 22;
         if (g >= (uint32_t)ng) fail("xe2 group");
         else x2[(size_t)g * 65536 + k].push_back((int)li);
+        if (im.xe2[4 * i + 1] != im.memo[1].beg[li] || im.xe2[4 * i + 2] != im.memo[1].beg[li + 1] ||
+            (im.xe2[4 * i + 3] & kIdMask) != im.memo[1].nrep[li])
+          fail("xe2 list range");
       }
     std::vector<std::vector<int>> x1(256);
     for (int b = 0; b < 256; b++) {
@@ -145,7 +148,11 @@ int main() {
       for (int b0 = 0; b0 < 42; b0++) {
         const uint32_t li1 = gi * 256 + b0;
         uint32_t nrep1 = 0;
-        if (list_of(im.memo[0], li1, &nrep1) != brute(g, {b0})) fail("depth 1");
+        const std::set<int> want1 = brute(g, {b0});
+        if (list_of(im.memo[0], li1, &nrep1) != want1) fail("depth 1");
+        const uint32_t *m1 = &im.mask1[(size_t)gi * 16];
+        if (((m1[b0 / 32] >> (b0 % 32)) & 1u) != !want1.empty()) fail("mask1 roots");
+        if (((m1[8 + b0 / 32] >> (b0 % 32)) & 1u) != (nrep1 > 0)) fail("mask1 reports");
         const bool in_x1 =
             std::find(x1[b0].begin(), x1[b0].end(), (int)li1) != x1[b0].end();
         if (in_x1 != (nrep1 > 0)) fail("xe1 membership");
