@@ -330,8 +330,18 @@ def main():
             f"{model:.2f} algorithmic bytes per input byte ({per_byte:.1f} group by group, "
             f"{per_byte_ref:.1f} for the reference algorithm); {st}")
         achieved = model * total / (kern * 1e-3) / 1e9
+        # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # on this workload and engine (profiles/), scaled by the copy calibration
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+        if os.path.exists(tpath):
+            t = json.load(open(tpath)).get(a.workload)
+            if t and t.get("engine") == main_s:
+                traffic = t["bytes_per_launch"]
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel="k_batched",
+                        frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
+                        traffic_gbs=round(traffic / (kern * 1e-3) / 1e9, 1) if traffic else None,
+                        kernel="k_batched",
                         work_items="all-group" if cross else "per-group",
                         algorithmic_bytes_per_input_byte=round(model, 3),
                         per_group_model_bytes_per_input_byte=round(per_byte, 3),

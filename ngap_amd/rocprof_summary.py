@@ -41,7 +41,7 @@ def pmc(db):
 
 def pmc_values(db, pattern):
     return _rows(db, "select counter_name, counter_value, duration from pmc_events where name "
-                     f"like '%{pattern}%' order by rowid")
+                     f"like '%{pattern}%' order by start, dispatch_id")
 
 
 def main():

@@ -76,6 +76,10 @@ def test_regex_patterns_multi_chunk(engine, seed):
                                   ("nonblockingpcgroups", "plain", 3, 1, 0),
                                   ("nonblockingr1groups", "batched", 1, 0, 8192),
                                   ("NAPgroups", "plain", 0, 0, 0)]]
+    # the reference's scheduling knobs never change the report set
+    cfgs += [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=3,
+                  data_buffer_fetch_size=25600, add_aas_start=256, add_aas_interval=256,
+                  active_threshold=t) for s, t in (("plain", 16), ("batched", 0))]
     # one group through the cross-group index (forced)
     cfgs += [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=d,
                   unique=1, cross_groups=2, window=w) for d, w in ((2, 0), (3, 777))]
