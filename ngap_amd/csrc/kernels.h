@@ -15,7 +15,7 @@ namespace ngapamd {
 
 struct DevImage {
   const GroupMeta *meta;
-  const int32_t *succ4;      // 4 per record (read as int4)
+  const int32_t *succ4;      // kRecWords per record (ids, then tests; two int4 loads)
   const uint32_t *cls;       // 8 per state
   const int32_t *sod;
   const uint32_t *beg[kMaxMemoDepth];

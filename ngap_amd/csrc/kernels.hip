@@ -85,6 +85,42 @@ NGAP_DEV uint64_t item_of(uint32_t tagged, uint32_t pos) {
   return ((uint64_t)pos << 32) | tagged;
 }
 
+// A 32-byte successor record: four tagged successor ids and their tests.
+struct Rec {
+  int4 ids;
+  uint4 tests;
+};
+
+NGAP_DEV Rec load_rec(const KParams &P, uint32_t r) {
+  const int4 *R = reinterpret_cast<const int4 *>(P.img.succ4) + (size_t)r * 2;
+  Rec x;
+  x.ids = R[0];
+  const int4 t = R[1];
+  x.tests = make_uint4((uint32_t)t.x, (uint32_t)t.y, (uint32_t)t.z, (uint32_t)t.w);
+  return x;
+}
+
+NGAP_DEV Rec empty_rec() {
+  Rec x;
+  x.ids = make_int4(-1, -1, -1, -1);
+  x.tests = make_uint4(0, 0, 0, 0);
+  return x;
+}
+
+// Class word a successor test needs (0 when the test word decides alone).
+NGAP_DEV uint32_t test_word(const KParams &P, uint32_t cls_base, bool live, int32_t e,
+                            uint32_t t, uint32_t b) {
+  if (!live || e < 0 || (t & (kTestLiteral | kTestAll))) return 0u;
+  return P.img.cls[(size_t)(cls_base + ((uint32_t)e & (uint32_t)kIdMask)) * 8 + (b >> 5)];
+}
+
+NGAP_DEV bool test_hit(bool live, int32_t e, uint32_t t, uint32_t word, uint32_t b) {
+  if (!live || e < 0) return false;
+  if (t & kTestLiteral) return (t & 255u) == b;
+  if (t & kTestAll) return true;
+  return (word >> (b & 31)) & 1u;
+}
+
 NGAP_DEV void push(const KParams &P, WaveState &W, bool pred, uint64_t item) {
   uint64_t m = wballot(pred);
   if (m == 0) return;
@@ -408,10 +444,10 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
                      uint32_t &p, int &depth, const LaneGroup &lg) {
   const uint32_t p1 = p + 1;
   uint32_t b = 0;
-  int4 rec = make_int4(-1, -1, -1, -1);
+  Rec rec = empty_rec();
   if (valid) {
     b = P.input[p1];
-    rec = reinterpret_cast<const int4 *>(P.img.succ4)[lg.rec_base + v];
+    rec = load_rec(P, lg.rec_base + v);
   }
   W.items += __popcll(wballot(valid));
   bool has_next = false;
@@ -420,26 +456,23 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
   const bool can_priv = P.priv_limit < 0 || depth < P.priv_limit;
   while (wballot(more)) {
     int32_t link = -1;
-    const int32_t es[4] = {rec.x, rec.y, rec.z, rec.w};
-    // all loads of this record first: the successors' class words and the
+    const int32_t es[4] = {rec.ids.x, rec.ids.y, rec.ids.z, rec.ids.w};
+    const uint32_t ts[4] = {rec.tests.x, rec.tests.y, rec.tests.z, rec.tests.w};
+    // all loads of this record first: the class words its tests need and the
     // continuation record, then the wave-wide emits and pushes
     uint32_t cw[4];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      const int32_t e = es[s];
-      if (more && e <= -2) link = -(e + 2);
-      cw[s] = (more && e >= 0)
-                  ? P.img.cls[(size_t)(lg.cls_base + ((uint32_t)e & (uint32_t)kIdMask)) * 8 +
-                              (b >> 5)]
-                  : 0u;
+      if (more && es[s] <= -2) link = -(es[s] + 2);
+      cw[s] = test_word(P, lg.cls_base, more, es[s], ts[s], b);
     }
-    int4 next = make_int4(-1, -1, -1, -1);
-    if (more && link >= 0) next = reinterpret_cast<const int4 *>(P.img.succ4)[link];
+    Rec next = empty_rec();
+    if (more && link >= 0) next = load_rec(P, (uint32_t)link);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int32_t e = es[s];
       const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-      const bool hit = (cw[s] >> (b & 31)) & 1u;
+      const bool hit = test_hit(more, e, ts[s], cw[s], b);
       const uint64_t it = item_of(u | lg.gtag, p1);
       emit(P, W, hit && (e & kFlagReport), it);
       bool keep = hit && !(e & kFlagDeadEnd) && (p1 + 1 < W.cend);
@@ -722,24 +755,26 @@ __global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
     for (uint32_t i = wave * kWave; i < n; i += kBlockThreads) {
       const uint32_t idx = i + W.lane;
       bool more = idx < n;
-      int4 rec = make_int4(-1, -1, -1, -1);
-      if (more) rec = reinterpret_cast<const int4 *>(P.img.succ4)[W.rec_base + cur[idx]];
+      Rec rec = empty_rec();
+      if (more) rec = load_rec(P, W.rec_base + cur[idx]);
       while (wballot(more)) {
         int32_t link = -1;
-        const int32_t es[4] = {rec.x, rec.y, rec.z, rec.w};
+        const int32_t es[4] = {rec.ids.x, rec.ids.y, rec.ids.z, rec.ids.w};
+        const uint32_t ts[4] = {rec.tests.x, rec.tests.y, rec.tests.z, rec.tests.w};
 #pragma unroll
         for (int s2 = 0; s2 < 4; s2++) {
           const int32_t e = es[s2];
           if (more && e <= -2) link = -(e + 2);
           const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
-          bool hit = more && e >= 0 && class_has(P, W.cls_base, u, b);
+          const bool hit =
+              test_hit(more, e, ts[s2], test_word(P, W.cls_base, more, e, ts[s2], b), b);
           emit(P, W, hit && (e & kFlagReport), item_of(u | W.gtag, p1));
           bool keep = hit && !(e & kFlagDeadEnd);
           if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u | W.gtag, p1));
           block_append(P, W, ncount, nxt, keep, u);
         }
         more = more && link >= 0;
-        if (more) rec = reinterpret_cast<const int4 *>(P.img.succ4)[link];
+        if (more) rec = load_rec(P, (uint32_t)link);
       }
     }
     // all-input states matched at p + 1

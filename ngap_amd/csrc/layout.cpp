@@ -115,29 +115,49 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
       tag[v] = t;
     }
 
+    // successor test words: a one-byte class is tested by comparison, an
+    // all-byte class always matches, any other class reads its class word
+    std::vector<int32_t> test(n, 0);
+    for (int v = 0; v < n; v++) {
+      int cnt = 0, last = 0;
+      for (int w = 0; w < 8; w++) {
+        const uint32_t x = hg.sets[(size_t)v * 8 + w];
+        if (x) last = w * 32 + 31 - __builtin_clz(x);
+        cnt += __builtin_popcount(x);
+      }
+      if (cnt == 1) test[v] = (int32_t)(kTestLiteral | (uint32_t)last);
+      else if (cnt == 256) test[v] = (int32_t)kTestAll;
+    }
+
     GroupMeta gm{};
-    gm.rec_base = (uint32_t)(img.succ4.size() / 4);
+    gm.rec_base = (uint32_t)(img.succ4.size() / kRecWords);
     gm.cls_base = (uint32_t)(img.cls.size() / 8);
     gm.num_states = n;
     gm.num_aas = hg.num_aas;
     // records: n primary records, continuation records appended
-    size_t prim = img.succ4.size();
-    img.succ4.resize(prim + (size_t)n * 4, -1);
+    auto new_records = [&](size_t count) {
+      const size_t at = img.succ4.size();
+      img.succ4.resize(at + count * kRecWords, -1);
+      for (size_t r = 0; r < count; r++)
+        for (int i = 4; i < 8; i++) img.succ4[at + r * kRecWords + i] = 0;
+      return at;
+    };
+    const size_t prim = new_records((size_t)n);
     for (int v = 0; v < n; v++) {
       const auto &s = succ[v];
-      size_t rec = prim + (size_t)v * 4;
+      size_t rec = prim + (size_t)v * kRecWords;
       size_t k = 0;
       for (;;) {
         size_t left = s.size() - k;
-        if (left <= 4) {
-          for (size_t i = 0; i < left; i++) img.succ4[rec + i] = tag[s[k + i]];
-          break;
+        const size_t here = left <= 4 ? left : 3;
+        for (size_t i = 0; i < here; i++) {
+          img.succ4[rec + i] = tag[s[k + i]];
+          img.succ4[rec + 4 + i] = test[s[k + i]];
         }
-        for (size_t i = 0; i < 3; i++) img.succ4[rec + i] = tag[s[k + i]];
+        if (left <= 4) break;
         k += 3;
-        size_t next = img.succ4.size();
-        img.succ4[rec + 3] = -(int32_t)(next / 4) - 2;
-        img.succ4.resize(next + 4, -1);
+        const size_t next = new_records(1);
+        img.succ4[rec + 3] = -(int32_t)(next / kRecWords) - 2;
         rec = next;
       }
     }
@@ -250,7 +270,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
       err = "memo tables too large (2^31 entries); use a smaller precompute depth";
       return false;
     }
-  if (img.succ4.size() / 4 >= (size_t)INT32_MAX / 2) {
+  if (img.succ4.size() / kRecWords >= (size_t)INT32_MAX / 2) {
     err = "automata image too large";
     return false;
   }

@@ -5,11 +5,15 @@
 // 128-248, group_graph.h:6-109, precompute_table.h:8-100) with flat arrays,
 // one allocation per kind for all groups:
 //
-//  * succ4  int32[4] per record.  Record v (< n) holds the successors of
-//    state v that are NOT all-input states, each tagged with the successor's
-//    flags; -1 pads, a value <= -2 continues at record -(value+2) (appended
-//    after the group's n records) when a state has more than 4 such
-//    successors.  One 16-byte load yields a state's whole successor list.
+//  * succ4  int32[8] per 32-byte record.  Words 0-3 of record v (< n) hold
+//    the successors of state v that are NOT all-input states, each tagged
+//    with the successor's flags; -1 pads, a value <= -2 continues at record
+//    -(value+2) (appended after the group's n records) when a state has more
+//    than 4 such successors.  Words 4-7 hold each successor's test: a class
+//    of one byte b is kTestLiteral | b (compared, no class word read), the
+//    class of all 256 bytes is kTestAll, any other class is 0 (read its class
+//    word).  One 32-byte record (half a cache line) yields a state's
+//    successor list and, for literal successors, their tests.
 //    All-input successors are dropped: an all-input state is enabled at every
 //    position anyway and enters as a root there, so the reference's re-adding
 //    it is a duplicate (kernel_ngap_OA.cu re-tests it; the oracle's set
@@ -39,6 +43,9 @@ constexpr int32_t kFlagReport = 1 << 22;    // state reports when matched
 constexpr int32_t kFlagDeadEnd = 1 << 23;   // no non-all-input successors
 constexpr int32_t kFlagDedup = 1 << 24;     // reachable along >= 2 edges
 constexpr int kMaxMemoDepth = 3;
+constexpr int kRecWords = 8;                  // int32 words per successor record
+constexpr uint32_t kTestLiteral = 1u << 31;   // successor test: one byte (bits 0-7)
+constexpr uint32_t kTestAll = 1u << 30;       // successor test: every byte
 
 struct HostGroup {  // what the C ABI hands in for one group
   int32_t num_states = 0;
@@ -83,7 +90,7 @@ struct Image {
   int32_t num_groups = 0;
   int32_t memo_depth = 0;
   std::vector<GroupMeta> meta;
-  std::vector<int32_t> succ4;   // 4 per record
+  std::vector<int32_t> succ4;   // kRecWords per record
   std::vector<uint32_t> cls;    // 8 per state
   std::vector<int32_t> sod;     // tagged SOD ids
   MemoLevel memo[kMaxMemoDepth];// memo[0] = depth 1 (always built)

@@ -171,7 +171,10 @@ int launch_slot(ngap_server *s, Slot &sl) {
   P.spill_cap = kServeSpill;
   attach_ctx(P, sl.ctx);
   sl.P = P;
-  sl.blocks = s->blocks_per_launch;
+  // a small launch takes only the workgroups its work items can feed (four
+  // waves each), so short requests do not pay for a full persistent grid
+  sl.blocks = std::max<uint32_t>(
+      1, std::min<uint32_t>(s->blocks_per_launch, ((uint32_t)sl.tasks.size() + 3) / 4));
   HIP_TRY(launch_reset(sl.ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
                        P.num_tasks, P.overflow_count, 0));
   HIP_TRY(launch_batched(sl.ctx.stream, P, sl.blocks, lds_bytes()));

@@ -181,6 +181,46 @@ int main() {
         }
       }
     }
+    // successor records: each state's non-all-input successors with their
+    // flags and tests, through continuation records
+    for (int gi = 0; gi < ng; gi++) {
+      const G &g = gs[gi];
+      std::vector<char> aas(g.n, 0);
+      for (int v : g.aas) aas[v] = 1;
+      const uint32_t base = im.meta[gi].rec_base;
+      if (im.meta[gi].cls_base + (uint32_t)g.n > im.cls.size() / 8) fail("cls base");
+      for (int v = 0; v < g.n; v++) {
+        std::set<int> want;
+        for (int e = g.row[v]; e < g.row[v + 1]; e++)
+          if (!aas[g.col[e]]) want.insert(g.col[e]);
+        std::set<int> got;
+        int64_t r = base + v;
+        for (int hops = 0; r >= 0 && hops < 100; hops++) {
+          const int32_t *rec = &im.succ4[(size_t)r * kRecWords];
+          int64_t next = -1;
+          for (int s = 0; s < 4; s++) {
+            const int32_t e = rec[s];
+            if (e <= -2) { next = -(int64_t)e - 2; continue; }
+            if (e < 0) continue;
+            const int u = e & kIdMask;
+            got.insert(u);
+            int cnt = 0;
+            for (int b = 0; b < 256; b++) cnt += has(g, u, b);
+            const uint32_t t = (uint32_t)rec[4 + s];
+            if (cnt == 1) {
+              if (!(t & kTestLiteral) || !has(g, u, (int)(t & 255))) fail("literal test");
+            } else if (cnt == 256) {
+              if (t != kTestAll) fail("all-byte test");
+            } else if (t != 0) {
+              fail("class test");
+            }
+            if (((e & kFlagReport) != 0) != ((g.attrs[u] & 2) != 0)) fail("report flag");
+          }
+          r = next;
+        }
+        if (got != want) fail("successor set");
+      }
+    }
     // start-of-data pairs: every group's entries, in group order
     size_t at = 0;
     for (int gi = 0; gi < ng; gi++)
