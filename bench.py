@@ -292,7 +292,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="learned",
                     choices=["learned", "batch1024", "regex20k", "literal64"])
-    ap.add_argument("--depth", type=int, default=2)
+    ap.add_argument("--depth", type=int, default=3,
+                    help="--precompute-depth of both engines (the reference's best setting is 3)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned schedule")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--also", default="batch1024",

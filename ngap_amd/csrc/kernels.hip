@@ -191,6 +191,18 @@ NGAP_DEV void refill_from_spill(const KParams &P, WaveState &W) {
 // position is the chunk's last.
 NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg,
                       uint32_t pos, uint32_t gtag, const int32_t *list) {
+  if (wballot(cnt > 1u) == 0) {
+    // at most one entry per lane (the common case): no prefix sum
+    const bool ok = cnt == 1u;
+    if (wballot(ok) == 0) return;
+    const int32_t e = ok ? list[beg & 0x7fffffffu] : 0;
+    const uint64_t it = item_of(((uint32_t)e & (uint32_t)kIdMask) | gtag, pos);
+    emit(P, W, ok && (e & kFlagReport), it);
+    bool keep = ok && !(beg >> 31) && !(e & kFlagDeadEnd) && (pos + 1 < W.cend);
+    if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
+    push(P, W, keep, it);
+    return;
+  }
   uint32_t incl = wave_incl_scan(cnt, W.lane);
   uint32_t total = __shfl(incl, 63, 64);
   if (total == 0) return;
@@ -307,6 +319,10 @@ NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) 
 // prefix sums held in the lanes).
 template <class F>
 NGAP_DEV void for_entries(const WaveState &W, uint32_t e0, uint32_t ecnt, F &&f) {
+  if (wballot(ecnt > 1u) == 0) {  // at most one entry per lane: its own
+    f(ecnt == 1u, e0, W.lane);
+    return;
+  }
   const uint32_t incl = wave_incl_scan(ecnt, W.lane);
   const uint32_t total = __shfl(incl, 63, 64);
   for (uint32_t base = 0; base < total; base += kWave) {
