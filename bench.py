@@ -337,7 +337,7 @@ def main():
         tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
         if os.path.exists(tpath):
             t = json.load(open(tpath)).get(a.workload)
-            if t and t.get("engine") == main_s:
+            if t and t.get("engine") == main_s and t.get("precompute_depth") == a.depth:
                 traffic = t["bytes_per_launch"]
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,

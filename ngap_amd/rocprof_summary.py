@@ -6,7 +6,8 @@
 
 `kernels` prints one row per kernel name (dispatches, average / min / max /
 total duration in ns); `pmc` one row per (kernel, counter) with the average
-counter value per dispatch (FETCH_SIZE and WRITE_SIZE are in KiB).  With
+counter value per dispatch, summed over the rows a dispatch has (one per XCC
+or shader engine for some counters; FETCH_SIZE and WRITE_SIZE are in KiB).  With
 --last, the last N dispatches of the kernel whose name contains KERNEL (in
 dispatch order) are summarised on their own: the timed steps of a benchmark
 that launches the same kernel for other purposes first.
@@ -34,14 +35,21 @@ def kernel_durations(db, pattern):
                                     f"'%{pattern}%' order by start")]
 
 
+# A counter may come in several rows per dispatch (one per XCC / shader
+# engine); rows are summed per dispatch first.
+_PER_DISPATCH = ("select name, dispatch_id, counter_name, sum(counter_value) as v, "
+                 "avg(duration) as d, min(start) as s from pmc_events "
+                 "group by dispatch_id, counter_name")
+
+
 def pmc(db):
-    return _rows(db, "select name, counter_name, count(*), avg(counter_value), avg(duration) "
-                     "from pmc_events group by name, counter_name")
+    return _rows(db, f"select name, counter_name, count(*), avg(v), avg(d) from ({_PER_DISPATCH}) "
+                     "group by name, counter_name")
 
 
 def pmc_values(db, pattern):
-    return _rows(db, "select counter_name, counter_value, duration from pmc_events where name "
-                     f"like '%{pattern}%' order by start, dispatch_id")
+    return _rows(db, f"select counter_name, v, d from ({_PER_DISPATCH}) where name "
+                     f"like '%{pattern}%' order by s, dispatch_id")
 
 
 def main():
