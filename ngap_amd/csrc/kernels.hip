@@ -383,29 +383,59 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
       });
     }
   }
+  if (P.memo_depth >= 3) {
+    // positions with three bytes left: depth-3 lists of every group from the
+    // 3-gram index, plus depth-2 reports where the 2-gram has any; the two
+    // positions before the chunk end use depth 2 and 1 as above
+    const bool rep2 = k >= 3 && ((P.img.x2rep[((b0 << 8) | b1) >> 5] >> (b1 & 31)) & 1u);
+    if (k != 2 && !rep2) ec = 0;
+    if (wballot(ec != 0)) {
+      for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
+        const uint32_t pos = __shfl(p, (int)o, 64);
+        const uint32_t ko = __shfl(k, (int)o, 64);
+        uint32_t beg = 0, cnt = 0, tag = 0;
+        if (ok) {
+          const uint4 e = reinterpret_cast<const uint4 *>(P.img.xe2)[idx];
+          tag = (e.w >> 22) << 22;
+          beg = e.y;
+          cnt = ko == 2 ? e.z - e.y : (e.w & (uint32_t)kIdMask);
+          if (ko != 2) beg |= kReportOnly;
+        }
+        flatten(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
+      });
+    }
+    uint32_t t0 = 0, tc = 0;
+    if (k >= 3) {
+      const uint32_t key3 = (b0 << 16) | (b1 << 8) | b2;
+      t0 = P.img.xr3[key3];
+      tc = P.img.xr3[key3 + 1] - t0;
+    }
+    if (wballot(tc != 0) == 0) return;
+    for_entries(W, t0, tc, [&](bool ok, uint32_t idx, uint32_t o) {
+      const uint32_t pos = __shfl(p, (int)o, 64);
+      uint32_t beg = 0, cnt = 0, tag = 0;
+      if (ok) {  // (begin in xl3, length | group << 22)
+        const uint2 e = reinterpret_cast<const uint2 *>(P.img.xe3)[idx];
+        beg = e.x;
+        cnt = e.y & (uint32_t)kIdMask;
+        tag = (e.y >> 22) << 22;
+      }
+      flatten(P, W, cnt, beg, pos + 2, tag, P.img.xl3);
+    });
+    return;
+  }
   if (wballot(ec != 0) == 0) return;
   for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
     const uint32_t pos = __shfl(p, (int)o, 64);
-    const uint32_t ko = __shfl(k, (int)o, 64);
-    const uint32_t c2 = __shfl(b2, (int)o, 64);
     uint32_t beg = 0, cnt = 0, tag = 0;
-    int32_t li3 = -1;
     if (ok) {
       // (list index, begin, end, reporting entries | group << 22)
       const uint4 e = reinterpret_cast<const uint4 *>(P.img.xe2)[idx];
       tag = (e.w >> 22) << 22;
       beg = e.y;
-      cnt = ko == 2 ? e.z - e.y : (e.w & (uint32_t)kIdMask);
-      if (ko != 2) beg |= kReportOnly;
-      if (ko >= 3) li3 = memo_rank(P.img.sel[2], (size_t)e.x * 8 + (c2 >> 5), c2 & 31);
-    }
-    uint32_t beg3 = 0, cnt3 = 0;
-    if (li3 >= 0) {
-      beg3 = P.img.beg[2][li3];
-      cnt3 = P.img.beg[2][li3 + 1] - beg3;
+      cnt = e.z - e.y;
     }
     flatten(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
-    if (wballot(cnt3 != 0)) flatten(P, W, cnt3, beg3, pos + 2, tag, P.img.list[2]);
   });
 }
 

@@ -12,6 +12,8 @@ uint64_t Image::bytes() const {
                sod.size() * 4;
   for (auto &m : memo)
     b += (m.beg.size() + m.nrep.size() + m.sel.size() + m.list.size()) * 4;
+  b += (mask1.size() + xr1.size() + xe1.size() + xmask1.size() + xr2.size() + xe2.size() +
+        xsod.size() + xr3.size() + xe3.size() + xl3.size() + x2rep.size()) * 4;
   return b;
 }
 
@@ -55,6 +57,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
   img.memo_depth = memo_depth;
   for (int d = 0; d < kMaxMemoDepth; d++) img.memo[d].beg.push_back(0);
   std::vector<uint32_t> key2, grp2;  // 2-gram and group of every depth-2 list
+  std::vector<uint32_t> key3, grp3;  // 3-gram and group of every depth-3 list
 
   for (int32_t g = 0; g < num_groups; g++) {
     const HostGroup &hg = groups[g];
@@ -189,6 +192,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     std::vector<std::vector<int32_t>> bucket(256);
     std::vector<std::vector<int32_t>> prev_lists;  // lists of previous depth
     for (int b = 0; b < 256; b++) prev_lists.push_back(l1[b]);
+    const size_t key2_base = key2.size();  // this group's depth-2 lists in key2
     for (int d = 2; d <= memo_depth; d++) {
       MemoLevel &m = img.memo[d - 1];
       // depth 2: 2048 pairs per group; depth 3: 8 pairs per depth-2 key
@@ -216,6 +220,9 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
             if (d == 2) {
               key2.push_back(((uint32_t)li << 8) | (uint32_t)b);
               grp2.push_back((uint32_t)g);
+            } else if (d == 3) {
+              key3.push_back((key2[key2_base + li] << 8) | (uint32_t)b);
+              grp3.push_back((uint32_t)g);
             }
             cur_lists.push_back(bucket[b]);
             push_list(m, bucket[b], tag);
@@ -264,6 +271,30 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
       e[2] = m2.beg[i + 1];
       e[3] = m2.nrep[i] | (grp2[i] << 22);  // nrep < 2^22: at most the group's states
     }
+  }
+  if (memo_depth >= 3) {
+    // 3-gram k -> every group's depth-3 list, lists copied in key order
+    const MemoLevel &m3 = img.memo[2];
+    img.xr3.assign((1u << 24) + 1, 0);
+    for (uint32_t k : key3) img.xr3[k + 1]++;
+    for (uint32_t k = 0; k < (1u << 24); k++) img.xr3[k + 1] += img.xr3[k];
+    std::vector<uint32_t> order(key3.size());
+    {
+      std::vector<uint32_t> at(img.xr3.begin(), img.xr3.end() - 1);
+      for (size_t i = 0; i < key3.size(); i++) order[at[key3[i]]++] = (uint32_t)i;
+    }
+    img.xe3.resize(key3.size() * 2);
+    img.xl3.reserve(m3.list.size());
+    for (size_t j = 0; j < order.size(); j++) {
+      const uint32_t li = order[j];
+      img.xe3[2 * j] = (uint32_t)img.xl3.size();
+      img.xe3[2 * j + 1] = (m3.beg[li + 1] - m3.beg[li]) | (grp3[li] << 22);
+      img.xl3.insert(img.xl3.end(), m3.list.begin() + m3.beg[li], m3.list.begin() + m3.beg[li + 1]);
+    }
+    // 2-grams with a reporting depth-2 list (their reports come from depth 2)
+    img.x2rep.assign(2048, 0);
+    for (size_t i = 0; i < key2.size(); i++)
+      if (img.memo[1].nrep[i]) img.x2rep[key2[i] >> 5] |= 1u << (key2[i] & 31);
   }
   for (auto &m : img.memo)
     if (m.list.size() >= (1ull << 31)) {

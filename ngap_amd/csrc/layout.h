@@ -105,7 +105,13 @@ struct Image {
   //       xr2[k+1]-1 of xe2, each 4 words (list index, list begin, list end,
   //       reporting entries | group << 22), one 16-byte load per entry;
   //   xsod: (tagged id, group) pairs of every group's start-of-data states.
-  std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod;
+  //   x3 (depth 3): 3-gram k -> every group's depth-3 list: entries xr3[k]
+  //       .. xr3[k+1]-1 of xe3, each (begin in xl3, length | group << 22),
+  //       the lists copied into xl3 in key order (one position reads one
+  //       offset pair and then contiguous entries and lists); x2rep marks the
+  //       2-grams that have a reporting depth-2 list (2048 words).
+  std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod, xr3, xe3, x2rep;
+  std::vector<int32_t> xl3;
   // statistics (for the learned scheduler's features and reports)
   std::vector<uint32_t> stat_edges, stat_aas, stat_sod, stat_report,
       stat_wide, stat_selfloop;

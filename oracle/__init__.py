@@ -27,7 +27,7 @@ class OracleStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "tested_non_aas", "matched_aas", "matched_advance", "successors", "reports",
         "enabled_items", "matched_items", "memo_list_items", "expanded_items",
-        "expanded_tests", "root_positions", "level2_lists")]
+        "expanded_tests", "root_positions", "level2_lists", "deepest_lists")]
 
 
 _lib = None
@@ -105,13 +105,13 @@ def algorithmic_bytes(stats, unit_bytes):
 
 def algorithmic_bytes_cross(stats, positions, memo_depth):
     """Bytes of the all-group engine (memo depth >= 2), which reads the
-    cross-group index once per position instead of scanning every group:
-    1 B of input and an 8 B offset pair per position; per non-empty depth-2
-    list an 8 B (list, group) entry and its 8 B offsets, plus an 8 B rank pair
-    at depth 3; then the same list entries, expanded items and reports as
+    cross-group index of the deepest level once per position instead of
+    scanning every group: 1 B of input and an 8 B offset pair per position;
+    per non-empty deepest list its index entry (16 B at depth 2, 8 B at depth
+    3); then the same list entries, expanded items and reports as
     algorithmic_bytes."""
-    per_list = 16 + (8 if memo_depth >= 3 else 0)
-    return (9 * positions + per_list * stats["level2_lists"] + 4 * stats["memo_list_items"]
+    per_list = 16 if memo_depth == 2 else 8
+    return (9 * positions + per_list * stats["deepest_lists"] + 4 * stats["memo_list_items"]
             + 16 * stats["expanded_items"] + 4 * stats["expanded_tests"]
             + 8 * stats["reports"])
 

@@ -66,6 +66,9 @@ struct oracle_stats {
   uint64_t level2_lists;     // memo depth >= 2: positions where a state one
                              // step below an all-input root matched (a
                              // non-empty depth-2 list)
+  uint64_t deepest_lists;    // positions where a state k-1 steps below an
+                             // all-input root matched (a non-empty list of the
+                             // deepest memo level k)
 };
 
 // Runs the oracle.  `unit_mask` (nullable, length num_groups*num_seg, index
@@ -117,7 +120,7 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
       const uint8_t b = input[p];
       nxt.clear();
       st.enabled_items += cur.size();
-      bool root_here = false, list2_here = false;
+      bool root_here = false, list2_here = false, deep_here = false;
       for (const SD &vd : cur) {
         const int32_t v = vd.first, d = vd.second;
         if (!is_aas[v]) st.tested_non_aas++;
@@ -134,6 +137,7 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
         const bool materialised = d >= K - 1;
         if (d == K - 1) st.memo_list_items++;
         if (d == 1) list2_here = true;
+        if (d == K - 1) deep_here = true;
         if (p + 1 < end) {
           st.matched_advance++;
           st.successors += (uint64_t)(g.row[v + 1] - g.row[v]);
@@ -149,6 +153,7 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
       }
       st.root_positions += root_here;
       st.level2_lists += list2_here;
+      st.deepest_lists += deep_here;
       for (int i = 0; i < g.num_aas; i++) nxt.emplace_back(g.aas[i], 0);
       merge(nxt);
       cur.swap(nxt);
@@ -177,6 +182,7 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
     tot.expanded_tests += per_stats[u].expanded_tests;
     tot.root_positions += per_stats[u].root_positions;
     tot.level2_lists += per_stats[u].level2_lists;
+    tot.deepest_lists += per_stats[u].deepest_lists;
   }
   // compareResult order: by offset (low word), then by id (high word)
   std::sort(all.begin(), all.end(), [](uint64_t a, uint64_t b) {

@@ -172,11 +172,26 @@ int main() {
           }
           if (list_of(im.memo[1], li2, nullptr) != want2) fail("depth 2 list");
           if (xe.size() != 1 || xe[0] != li2) fail("xe2 entry");
+          uint32_t nrep2 = 0;
+          list_of(im.memo[1], li2, &nrep2);
+          if (nrep2 && !((im.x2rep[key >> 5] >> (key & 31)) & 1u)) fail("x2rep");
           for (int b2 = 0; b2 < 42; b2++) {
             const std::set<int> want3 = brute(g, {b0, b1, b2});
             const int li3 = rank_lookup(im.memo[2].sel, (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
             if (want3.empty() != (li3 < 0)) fail("depth 3 presence");
             else if (li3 >= 0 && list_of(im.memo[2], li3, nullptr) != want3) fail("depth 3 list");
+            // the 3-gram cross-group index: this group's entry, its list
+            const uint32_t k3 = (key << 8) | (uint32_t)b2;
+            int found = 0;
+            for (uint32_t i = im.xr3[k3]; i < im.xr3[k3 + 1]; i++) {
+              if ((im.xe3[2 * i + 1] >> 22) != (uint32_t)gi) continue;
+              found++;
+              std::set<int> got;
+              const uint32_t beg = im.xe3[2 * i], cnt = im.xe3[2 * i + 1] & kIdMask;
+              for (uint32_t j = beg; j < beg + cnt; j++) got.insert(im.xl3[j] & kIdMask);
+              if (got != want3) fail("xl3 list");
+            }
+            if (found != (want3.empty() ? 0 : 1)) fail("xe3 entry");
           }
         }
       }

@@ -131,6 +131,7 @@ def test_memo_work_counts_on_a_closed_form(k):
     assert st["root_positions"] == n               # every 'a'
     if k >= 2:                                     # (counted for memo depth >= 2)
         assert st["level2_lists"] == n             # every 'b' (one step below a root)
+    assert st["deepest_lists"] == n                # 'a', 'b' or 'c' for k = 1, 2, 3
 
 
 def test_memo_depth_restarts_at_all_input_successors():
