@@ -106,6 +106,8 @@ typedef struct {
                                    all groups when there are several and the
                                    precompute depth is >= 2, 2 = also for one
                                    group, 0 = one group per work item */
+  uint32_t ring_capacity;       /* batched: LDS worklist items per wave, a power
+                                   of two in [64, 2048], 0 = 512 */
 } ngap_run_opts;
 
 typedef struct {
@@ -137,10 +139,11 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats)
 
 /* Autotuner of the batched engine (the reference picks these knobs by hand
  * per application, code/scripts/configs/exec_config_ngap_groups_best): times
- * candidate work-item windows, persistent grid sizes and worklist spill
- * capacities on the loaded input by coordinate descent (at most max_runs
- * timed runs) and writes the fastest into *best: a copy of *base, batched,
- * with fallback on, window / persistent_blocks / spill_capacity set.  Report
+ * candidate work-item windows, LDS worklist capacities, persistent grid sizes
+ * and worklist spill capacities on the loaded input by coordinate descent (at
+ * most max_runs timed runs) and writes the fastest into *best: a copy of
+ * *base, batched, with fallback on, window / ring_capacity /
+ * persistent_blocks / spill_capacity set.  Report
  * sets do not depend on these knobs; the chunk length is part of the input's
  * meaning (matches do not cross chunks) and is not tuned. */
 int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,

@@ -69,6 +69,7 @@ class RunOpts(C.Structure):
         ("num_tasks", C.c_uint32),
         ("fallback", C.c_int32),
         ("cross_groups", C.c_int32),
+        ("ring_capacity", C.c_uint32),
     ]
 
 

@@ -71,7 +71,16 @@ void RunCtx::drop_graph() {
   gkey.clear();
 }
 
-uint32_t lds_bytes() { return 16 + (kBlockThreads / kWave) * wave_lds_bytes(kRing, kRbuf, kDc); }
+uint32_t lds_bytes(uint32_t ring) {
+  return 16 + (kBlockThreads / kWave) * wave_lds_bytes(ring, kRbuf, kDc);
+}
+
+// ring_capacity option -> LDS ring entries (0 = default); 0 when invalid
+uint32_t ring_of(const ngap_run_opts &o) {
+  const uint32_t r = o.ring_capacity ? o.ring_capacity : kRing;
+  if (r < kMinRing || r > kMaxRing || (r & (r - 1))) return 0;
+  return r;
+}
 
 bool map_algorithm(int algorithm, int depth_req, int loaded_depth, Mapping &m,
                    std::string &err) {
@@ -143,7 +152,7 @@ KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_op
   P.dedup = o.unique;
   P.priv_limit = mp.priv_limit;
   P.report_off = o.report_off;
-  P.ring_cap = kRing;
+  P.ring_cap = o.schedule == NGAP_SCHED_BATCHED && ring_of(o) ? ring_of(o) : kRing;
   P.rbuf_cap = kRbuf;
   P.dc_cap = kDc;
   P.root_batch = kWave;
@@ -161,11 +170,13 @@ void attach_ctx(KParams &P, RunCtx &ctx) {
   P.report_cap = ctx.report_cap;
 }
 
-int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks) {
-  int per_cu = eng->batched_per_cu;
+int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks,
+                      uint32_t ring) {
+  const int slot = 31 - __builtin_clz(ring);  // log2, ring in [64, 2048]
+  int per_cu = eng->batched_per_cu[slot];
   if (per_cu == 0) {
-    HIP_TRY(batched_occupancy(&per_cu, lds_bytes()));
-    const_cast<ngap_engine *>(eng)->batched_per_cu = per_cu;
+    HIP_TRY(batched_occupancy(&per_cu, lds_bytes(ring)));
+    const_cast<ngap_engine *>(eng)->batched_per_cu[slot] = per_cu;
   }
   if (per_cu < 1) per_cu = 1;
   *blocks = requested ? requested : (uint32_t)(per_cu * eng->num_cus);
@@ -177,7 +188,7 @@ int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *bloc
 // graph holds kernels only (the counters are cleared by k_reset).
 static int launch_batched_ctx(RunCtx &ctx, const KParams &P, uint32_t blocks, uint32_t n_flags,
                               bool keep_reports, bool use_graph) {
-  const uint32_t lds = lds_bytes();
+  const uint32_t lds = lds_bytes(P.ring_cap);
   if (use_graph) {
     std::vector<unsigned char> key(sizeof P + 12);
     memcpy(key.data(), &P, sizeof P);
@@ -232,7 +243,7 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   P.task_items = ctx.task_items.as<unsigned long long>();
   P.overflow = ctx.overflow.as<uint32_t>();
   uint32_t blocks = 0;
-  int rc = persistent_blocks(eng, o.persistent_blocks, &blocks);
+  int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap);
   if (rc) return rc;
   P.spill_cap = o.spill_capacity ? o.spill_capacity : 65536;
   HIP_TRY(ctx.spill.reserve((size_t)blocks * P.waves_per_block * P.spill_cap * 8));
@@ -454,6 +465,7 @@ int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
   const uint32_t G = (uint32_t)eng->img.num_groups, S = eng->num_seg;
   const uint32_t units = G * S;
   const bool blocking = o.algorithm == NGAP_ALGO_BLOCKING;
+  P.ring_cap = kRing;  // these launches size their LDS for the default ring
   uint32_t spill = o.spill_capacity;
   if (!spill) {
     // default budget: 8 GiB of spill (or frontier) for the whole grid, at
@@ -540,7 +552,7 @@ int ngap_create(int32_t device, ngap_engine **out) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   e->num_cus = prop.multiProcessorCount;
-  HIP_TRY(set_lds_limit(lds_bytes()));
+  HIP_TRY(set_lds_limit(lds_bytes(kMaxRing)));
   *out = e.release();
   return NGAP_OK;
 }
@@ -637,6 +649,8 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   std::string err;
   if (!map_algorithm(o.algorithm, o.precompute_depth, eng->img.memo_depth, mp, err))
     return ngap_fail(NGAP_ERR_INVALID, err);
+  if (!ring_of(o))
+    return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
   const uint64_t units = (uint64_t)eng->img.num_groups * eng->num_seg;
   if (units >= (1u << 31)) return ngap_fail(NGAP_ERR_INVALID, "too many (group, chunk) units");
   RunCtx &ctx = eng->ctx;
@@ -660,7 +674,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   uint32_t waves = 0;
   if (kind == 1) {
     uint32_t blocks = 0;
-    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks);
+    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap);
     if (rc) return rc;
     waves = blocks * P.waves_per_block;
   }
@@ -704,9 +718,6 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
   o.fallback = 1;
   o.tasks = nullptr;
   o.num_tasks = 0;
-  int per_cu = 0;
-  HIP_TRY(batched_occupancy(&per_cu, lds_bytes()));
-  const uint32_t full = (uint32_t)std::max(1, per_cu) * (uint32_t)eng->num_cus;
   const uint32_t L = (uint32_t)eng->chunk_len;
   int runs = 0;
   double best_t = 1e300;
@@ -725,11 +736,21 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
   };
   int rc = timeit(cur, best_t);
   if (rc) return rc;
-  std::vector<uint32_t> windows = {256, 512, 1024, 2048, 4096, 8192, 16384};
-  std::vector<uint32_t> grids = {full / 4, full / 2, (full * 3) / 4, full};
-  std::vector<uint32_t> spills = {8192, 16384, 262144};
-  for (int axis = 0; axis < 3; axis++) {
-    const std::vector<uint32_t> &vals = axis == 0 ? windows : axis == 1 ? grids : spills;
+  const std::vector<uint32_t> windows = {256, 512, 1024, 2048, 4096, 8192, 16384};
+  const std::vector<uint32_t> rings = {128, 256, 1024};
+  const std::vector<uint32_t> spills = {8192, 16384, 262144};
+  for (int axis = 0; axis < 4; axis++) {
+    // the grid axis scales the full grid of the ring capacity picked so far
+    uint32_t full = 0;
+    if (axis == 2) {
+      ngap_run_opts f = cur;
+      f.persistent_blocks = 0;
+      int rc2 = persistent_blocks(eng, 0, &full, ring_of(f));
+      if (rc2) return rc2;
+    }
+    const std::vector<uint32_t> grids = {full / 4, full / 2, (full * 3) / 4};
+    const std::vector<uint32_t> &vals =
+        axis == 0 ? windows : axis == 1 ? rings : axis == 2 ? grids : spills;
     ngap_run_opts pick = cur;
     for (uint32_t v : vals) {
       if (runs >= max_runs) break;
@@ -738,6 +759,8 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
         if (v > L) continue;
         c.window = v;
       } else if (axis == 1) {
+        c.ring_capacity = v;
+      } else if (axis == 2) {
         if (v == 0) continue;
         c.persistent_blocks = v;
       } else {

@@ -26,6 +26,7 @@ namespace ngapamd {
 
 // LDS capacities per wave (entries)
 constexpr uint32_t kRing = 512, kRbuf = 128, kDc = 256;
+constexpr uint32_t kMinRing = 64, kMaxRing = 2048;  // batched ring_capacity range
 
 struct DevBuf {
   void *p = nullptr;
@@ -73,14 +74,14 @@ struct Mapping {
 };
 
 bool map_algorithm(int algorithm, int depth_req, int loaded_depth, Mapping &m, std::string &err);
-uint32_t lds_bytes();
+uint32_t lds_bytes(uint32_t ring = kRing);
 
 }  // namespace ngapamd
 
 struct ngap_engine {
   int device = 0;
   int num_cus = 0;
-  int batched_per_cu = 0;  // k_batched occupancy (blocks per CU), 0 = not queried
+  int batched_per_cu[12] = {0};  // k_batched blocks per CU by log2(ring), 0 = not queried
   ngapamd::Image img;
   bool loaded = false;
   ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;
@@ -110,7 +111,9 @@ struct ngap_engine {
 namespace ngapamd {
 KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_opts &o);
 void attach_ctx(KParams &P, RunCtx &ctx);
-int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks);
+int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks,
+                      uint32_t ring = kRing);
+uint32_t ring_of(const ngap_run_opts &o);
 // Runs tasks on ctx (batched kernel), with the overflow fallback of o.
 int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> &tasks,
               uint64_t sched_id,

@@ -177,7 +177,7 @@ int launch_slot(ngap_server *s, Slot &sl) {
       1, std::min<uint32_t>(s->blocks_per_launch, ((uint32_t)sl.tasks.size() + 3) / 4));
   HIP_TRY(launch_reset(sl.ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
                        P.num_tasks, P.overflow_count, 0));
-  HIP_TRY(launch_batched(sl.ctx.stream, P, sl.blocks, lds_bytes()));
+  HIP_TRY(launch_batched(sl.ctx.stream, P, sl.blocks, lds_bytes(P.ring_cap)));
   HIP_TRY(hipEventRecord(sl.ctx.ev1, sl.ctx.stream));
   sl.bytes = total + report_cap * 8 + spill_bytes;
   s->inflight_bytes += sl.bytes;
@@ -265,7 +265,9 @@ int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
     return ngap_fail(NGAP_ERR_INVALID, err);
   HIP_TRY(hipSetDevice(eng->device));
   uint32_t blocks = 0;
-  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks);
+  if (!ring_of(*opts))
+    return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
+  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts));
   if (rc) return rc;
   // every launch gets the whole persistent grid: launches in flight share the
   // CUs, and a lone launch is not held to a fraction of the GPU

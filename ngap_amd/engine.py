@@ -61,7 +61,8 @@ class Engine:
                 unique=False, is_dup=False, active_threshold=20, report_off=False,
                 result_capacity=54619400, window=0, spill_capacity=0, persistent_blocks=0,
                 use_graph=False, data_buffer_fetch_size=128, add_aas_start=0,
-                add_aas_interval=1, tasks=None, fallback=False, cross_groups=1):
+                add_aas_interval=1, tasks=None, fallback=False, cross_groups=1,
+                ring_capacity=0):
         o = _lib.RunOpts()
         _lib.lib().ngap_default_opts(C.byref(o))
         if algorithm not in ALGORITHMS:
@@ -83,6 +84,7 @@ class Engine:
         o.add_aas_interval = int(add_aas_interval)
         o.fallback = int(bool(fallback))
         o.cross_groups = int(cross_groups)
+        o.ring_capacity = int(ring_capacity)
         if tasks is not None:
             # group -1 (or ALL_GROUPS) = a work item for every group
             arr = np.ascontiguousarray(np.asarray(tasks, np.int64).astype(np.uint32)

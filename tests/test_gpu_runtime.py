@@ -129,6 +129,25 @@ def test_schedule_cache_follows_input_and_groups(engine):
     assert np.array_equal(engine.reports(), want)
 
 
+def test_ring_capacity_changes_nothing_but_speed(engine):
+    groups, data = _workload(seed=26)
+    L = 1 << 13
+    want, _ = oracle.run(groups, data, L)
+    engine.load_groups(groups, 3)
+    engine.set_input(data, L)
+    for ring in (64, 128, 2048):
+        for x in (0, 1):
+            say("ring capacity", ring, "cross", x)
+            engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=3,
+                       ring_capacity=ring, cross_groups=x, spill_capacity=1 << 16)
+            assert np.array_equal(engine.reports(), want)
+    # the plain and blocking launches keep their own ring whatever the option says
+    engine.run(algorithm="blockinggroups", schedule="batched", ring_capacity=2048)
+    assert np.array_equal(engine.reports(), want)
+    with pytest.raises(NgapError):
+        engine.run(algorithm="nonblockingallgroups", schedule="batched", ring_capacity=96)
+
+
 def test_autotuner_picks_knobs_not_results(engine):
     groups, data = _workload(seed=24)
     L = 1 << 13
