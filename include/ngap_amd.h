@@ -108,6 +108,9 @@ typedef struct {
                                    group, 0 = one group per work item */
   uint32_t ring_capacity;       /* batched: LDS worklist items per wave, a power
                                    of two in [64, 2048], 0 = 512 */
+  int32_t task_traces;          /* batched: record the items of every work item
+                                   (ngap_get_task_items); a wave then finishes
+                                   one work item before it starts the next */
 } ngap_run_opts;
 
 typedef struct {
@@ -155,7 +158,7 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
 int ngap_get_reports(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out);
 
 /* Per-work-item item counts of the last batched run (frontier traces), in
- * the order of the work items. */
+ * the order of the work items; the run must have set task_traces. */
 int ngap_get_task_items(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out);
 /* The work items of the last batched run (the static schedule's choice when
  * none were given). */

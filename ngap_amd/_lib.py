@@ -70,6 +70,7 @@ class RunOpts(C.Structure):
         ("fallback", C.c_int32),
         ("cross_groups", C.c_int32),
         ("ring_capacity", C.c_uint32),
+        ("task_traces", C.c_int32),
     ]
 
 

@@ -44,6 +44,7 @@ struct KParams {
   DevImage img;
   const uint8_t *input;
   uint32_t chunk_len, num_seg, num_groups;
+  uint64_t chunk_magic;    // ceil(2^64 / chunk_len): divisibility test (chunk_len > 0)
   int32_t memo_depth;      // 1..3, d-gram roots
   int32_t is_dup;
   int32_t dedup;
@@ -70,11 +71,12 @@ struct KParams {
 };
 
 constexpr uint32_t kWave = 64;
+constexpr uint32_t kMaxOpen = 8;  // work items a wave overlaps (k_batched)
 constexpr uint32_t kBlockThreads = 256;
 
 // LDS bytes per wave for the given capacities
 NGAP_HD inline uint32_t wave_lds_bytes(uint32_t ring, uint32_t rbuf, uint32_t dc) {
-  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64 + 64 + 16) * 4 + 4;
+  return ring * 8 + rbuf * 8 + dc * 8 + (65 + 64 + 64 + 64 + 16 + kMaxOpen) * 4 + 4;
 }
 
 }  // namespace ngapamd

@@ -67,6 +67,8 @@ struct WaveState {
   uint32_t *sh_pos;   // 64
   uint32_t *sh_tag;   // 64
   uint32_t *mask1;    // 16: root bytes, then reporting depth-1 bytes (layout.h)
+  uint32_t *open;     // kMaxOpen: work items whose items may still be queued
+  uint32_t n_open;
   uint64_t *spill;
   uint32_t head, tail, rn, spill_n;
   uint32_t lane;
@@ -496,6 +498,11 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
     rec = load_rec(P, lg.rec_base + v);
   }
   W.items += __popcll(wballot(valid));
+  // is p1 the last position of its chunk (no successors enabled after it)?
+  // Engine runs have chunks of chunk_len bytes: a divisibility test, valid
+  // for items of any work item; request runs use the work item's bounds.
+  const bool last = P.chunk_len ? (uint64_t)(p1 + 1) * P.chunk_magic <= P.chunk_magic - 1
+                                : p1 + 1 >= W.cend;
   bool has_next = false;
   uint32_t nv = 0;
   bool more = valid;
@@ -521,7 +528,7 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
       const bool hit = test_hit(more, e, ts[s], cw[s], b);
       const uint64_t it = item_of(u | lg.gtag, p1);
       emit(P, W, hit && (e & kFlagReport), it);
-      bool keep = hit && !(e & kFlagDeadEnd) && (p1 + 1 < W.cend);
+      bool keep = hit && !(e & kFlagDeadEnd) && !last;
       if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
       const bool take = keep && !has_next && can_priv;
       if (take) {
@@ -542,8 +549,14 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
   }
 }
 
-NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t cstart,
-                         uint32_t cend) {
+NGAP_DEV void clear_dedup(const KParams &P, WaveState &W) {
+  for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
+  wsync();
+}
+
+// The current (group, chunk): group bases, tag and root masks in LDS.
+NGAP_DEV void set_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t cstart,
+                       uint32_t cend) {
   const GroupMeta gm = P.img.meta[g];
   W.g = g;
   W.gtag = g << 22;
@@ -551,20 +564,29 @@ NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t cs
   W.cls_base = gm.cls_base;
   W.cstart = cstart;
   W.cend = cend;
-  for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
   if (W.lane < 16) W.mask1[W.lane] = P.img.mask1[g * 16 + W.lane];
   wsync();
 }
 
 // A cross-group work item: every group's roots in one window of a chunk.
-NGAP_DEV void begin_cross(const KParams &P, WaveState &W, uint32_t cstart, uint32_t cend) {
+NGAP_DEV void set_cross(const KParams &P, WaveState &W, uint32_t cstart, uint32_t cend) {
   W.g = kAllGroups;
   W.gtag = 0;
   W.cstart = cstart;
   W.cend = cend;
-  for (uint32_t i = W.lane; i < P.dc_cap; i += kWave) W.dc[i] = ~0ull;
   if (W.lane < 8) W.mask1[8 + W.lane] = P.img.xmask1[W.lane];  // bytes with depth-1 reports
   wsync();
+}
+
+NGAP_DEV void begin_unit(const KParams &P, WaveState &W, uint32_t g, uint32_t cstart,
+                         uint32_t cend) {
+  clear_dedup(P, W);
+  set_unit(P, W, g, cstart, cend);
+}
+
+NGAP_DEV void begin_cross(const KParams &P, WaveState &W, uint32_t cstart, uint32_t cend) {
+  clear_dedup(P, W);
+  set_cross(P, W, cstart, cend);
 }
 
 NGAP_DEV void setup_wave(const KParams &P, WaveState &W, unsigned char *base,
@@ -577,6 +599,8 @@ NGAP_DEV void setup_wave(const KParams &P, WaveState &W, unsigned char *base,
   W.sh_pos = W.sh_beg + 64;
   W.sh_tag = W.sh_pos + 64;
   W.mask1 = W.sh_tag + 64;
+  W.open = W.mask1 + 16;
+  W.n_open = 0;
   W.spill = spill;
   W.head = W.tail = W.rn = W.spill_n = 0;
   W.lane = lane_id();
@@ -629,6 +653,117 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
     wsync();
     if (wballot(valid) == 0) {
       if (W.tail == W.head && W.spill_n == 0 && !roots_left) break;
+      continue;
+    }
+    expand(P, W, valid, v, p, depth, lg);
+  }
+}
+
+// Batched schedule, engine runs: the wave's stream of work items.  A wave
+// starts the next work item as soon as the current one has no positions left
+// and its queue runs low, so the tail of one work item's items overlaps the
+// roots of the next; at most kMaxOpen work items are open (items possibly
+// queued) at a time.  next(): 0 = a root batch [a, b), 1 = wait for the open
+// work items to finish, 2 = the queue is empty.
+struct TaskStream {
+  uint32_t cur = 0, end = 0;
+  bool cross = false;
+  NGAP_DEV int next(const KParams &P, WaveState &W, uint32_t &a, uint32_t &b) {
+    if (cur >= end) {
+      if (W.n_open >= kMaxOpen) return 1;
+      uint32_t t = 0;
+      if (W.lane == 0) t = atomicAdd(P.task_head, 1u);
+      t = __shfl(t, 0, 64);
+      if (t >= P.num_tasks) return 2;
+      const uint32_t ti = P.task_order ? P.task_order[t] : t;
+      const Task T = P.tasks[ti];
+      if (W.lane == 0) W.open[W.n_open] = ti;
+      W.n_open++;
+      cross = T.group == kAllGroups;
+      if (cross) {
+        set_cross(P, W, T.cstart, T.cend);
+        if (T.flags & kTaskSeedStart) seed_start_all(P, W);
+      } else {
+        set_unit(P, W, T.group, T.cstart, T.cend);
+        if (T.flags & kTaskSeedStart) {
+          const GroupMeta gm = P.img.meta[T.group];
+          seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
+        }
+      }
+      cur = T.begin;
+      end = T.end;
+    }
+    a = cur;
+    b = cur + kWave < end ? cur + kWave : end;
+    cur = b;
+    return 0;
+  }
+};
+
+// All queued items are done: the open work items are complete, and each is
+// marked for the overflow fallback when the wave dropped an item since they
+// opened.
+NGAP_DEV void close_open(const KParams &P, WaveState &W) {
+  if (W.n_open == 0) return;
+  wsync();
+  if (W.overflow) {
+    if (W.lane < W.n_open) P.overflow[W.open[W.lane]] = 1u;
+    if (W.lane == 0) atomicAdd(P.overflow_count, W.n_open);
+  }
+  W.n_open = 0;
+  W.overflow = false;
+  wsync();
+}
+
+// drain() over a TaskStream: items of several work items (and groups) share
+// the queue, so a lane looks up its item's group bases when it takes one.
+NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
+  TaskStream src;
+  bool valid = false;
+  uint32_t v = 0, p = 0;
+  int depth = 0;
+  LaneGroup lg{0u, 0u, 0u};
+  bool more = true;
+  for (;;) {
+    uint32_t qn = W.tail - W.head;
+    if (qn < kWave && W.spill_n > 0) {
+      refill_from_spill(P, W);
+      qn = W.tail - W.head;
+    }
+    if (qn < kWave && W.spill_n == 0 && more) {
+      uint32_t a, b;
+      const int st = src.next(P, W, a, b);
+      if (st == 0) {
+        if (src.cross) roots_cross(P, W, a, b);
+        else roots(P, W, a, b);
+        continue;
+      }
+      if (st == 2) more = false;
+    }
+    const uint64_t idle = wballot(!valid);
+    const uint32_t avail = W.tail - W.head;
+    const uint32_t nidle = __popcll(idle);
+    const uint32_t take = nidle < avail ? nidle : avail;
+    const uint32_t r = mask_rank(idle);
+    if (!valid && r < take) {
+      const uint64_t it = W.ring[(W.head + r) & (P.ring_cap - 1)];
+      v = (uint32_t)it & (uint32_t)kIdMask;
+      p = (uint32_t)(it >> 32);
+      valid = true;
+      depth = 0;
+      const uint32_t g = ((uint32_t)it >> 22) & 1023u;
+      const uint2 base = *reinterpret_cast<const uint2 *>(&P.img.meta[g]);
+      lg.gtag = g << 22;
+      lg.rec_base = base.x;
+      lg.cls_base = base.y;
+    }
+    W.head += take;
+    wsync();
+    if (wballot(valid) == 0) {
+      if (W.tail == W.head && W.spill_n == 0) {
+        close_open(P, W);
+        if (!more) break;
+      }
       continue;
     }
     expand(P, W, valid, v, p, depth, lg);
@@ -696,6 +831,14 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
   WaveState W;
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
+  if (P.chunk_len && !P.task_items) {
+    // engine runs without per-work-item traces: overlapping work items
+    clear_dedup(P, W);
+    drain_stream(P, W);
+    flush_reports(P, W);
+    if (W.lane == 0) atomicAdd(P.items, W.items);
+    return;
+  }
   for (;;) {
     uint32_t t = 0;
     if (W.lane == 0) t = atomicAdd(P.task_head, 1u);

@@ -240,7 +240,8 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   P.tasks = ctx.tasks.as<Task>();
   P.task_order = nullptr;
   P.num_tasks = T;
-  P.task_items = ctx.task_items.as<unsigned long long>();
+  P.task_items = o.task_traces ? ctx.task_items.as<unsigned long long>() : nullptr;
+  ctx.last_traces = o.task_traces != 0;
   P.overflow = ctx.overflow.as<uint32_t>();
   uint32_t blocks = 0;
   int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap);
@@ -497,6 +498,7 @@ int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
   ctx.last_report_off = o.report_off;
   ctx.have_run = true;
   ctx.last_tasks.clear();
+  ctx.last_traces = false;
   ctx.tasks_resident = false;
   ctx.resident_sched = 0;
   if (st) {
@@ -660,6 +662,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   KParams P = base_params(eng, mp, o);
   P.input = eng->d_input.as<uint8_t>();
   P.chunk_len = (uint32_t)eng->chunk_len;
+  P.chunk_magic = ~0ull / P.chunk_len + 1;  // wraps to 0 for 1-byte chunks, as the test needs
   P.num_seg = eng->num_seg;
   if (st) st->num_units = (uint32_t)units;
   if (o.schedule == NGAP_SCHED_PLAIN || o.algorithm == NGAP_ALGO_BLOCKING)
@@ -804,6 +807,8 @@ int ngap_get_task_items(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t 
   if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
   RunCtx &ctx = eng->ctx;
   if (!ctx.have_run) return ngap_fail(NGAP_ERR_STATE, "no run yet");
+  if (!ctx.last_traces)
+    return ngap_fail(NGAP_ERR_STATE, "the last run did not record work-item traces (task_traces)");
   *n_out = ctx.last_tasks.size();
   if (out && !ctx.last_tasks.empty()) {
     uint64_t n = std::min<uint64_t>(cap, ctx.last_tasks.size());

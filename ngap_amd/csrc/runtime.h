@@ -56,6 +56,7 @@ struct RunCtx {
   uint64_t report_cap = 0, last_count = 0;
   bool last_report_off = false, have_run = false;
   bool tasks_resident = false;  // `tasks` holds last_tasks
+  bool last_traces = false;     // the last run recorded items per work item
   uint64_t resident_sched = 0;  // schedule id of last_tasks (0 = unknown)
   std::vector<Task> last_tasks;
   hipGraph_t graph = nullptr;

@@ -80,7 +80,7 @@ def collect_traces(engine, data, chunk, num_groups, depth=2, window=WINDOW):
     S = len(data) // chunk
     tasks = fixed_tiling(num_groups, S, chunk, window)
     engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=depth,
-               unique=True, tasks=tasks, fallback=True)
+               unique=True, tasks=tasks, fallback=True, task_traces=True)
     items = engine.task_items().astype(np.float64)
     done = engine.tasks()
     return pairs_of(done, chunk, window), np.log1p(items).astype(np.float32)

@@ -64,7 +64,14 @@ def test_host_work_items(engine):
     assert st["num_tasks"] == len(tasks)
     assert np.array_equal(engine.reports(), want)
     assert engine.tasks().shape == (len(tasks), 4)
-    assert engine.task_items().shape == (len(tasks),)
+    with pytest.raises(NgapError):   # traces only when asked for
+        engine.task_items()
+    say("random tilings with traces")
+    st = engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=3,
+                    tasks=tasks, task_traces=True)
+    assert np.array_equal(engine.reports(), want)
+    items = engine.task_items()
+    assert items.shape == (len(tasks),) and int(items.sum()) == st["items"]
     # a gap or an overlap is rejected before anything runs
     say("bad tilings")
     with pytest.raises(NgapError):
