@@ -668,23 +668,34 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
 struct TaskStream {
   uint32_t cur = 0, end = 0;
   bool cross = false;
+  bool cross_mask = false;  // LDS holds the cross-group report mask
+  uint32_t slot0 = 0;       // lane 0: the next queue slot, claimed one work item ahead
+  NGAP_DEV void start(const KParams &P, const WaveState &W) {
+    if (W.lane == 0) slot0 = atomicAdd(P.task_head, 1u);
+  }
   NGAP_DEV int next(const KParams &P, WaveState &W, uint32_t &a, uint32_t &b) {
     if (cur >= end) {
       if (W.n_open >= kMaxOpen) return 1;
-      uint32_t t = 0;
-      if (W.lane == 0) t = atomicAdd(P.task_head, 1u);
-      t = __shfl(t, 0, 64);
+      const uint32_t t = __shfl(slot0, 0, 64);
       if (t >= P.num_tasks) return 2;
       const uint32_t ti = P.task_order ? P.task_order[t] : t;
       const Task T = P.tasks[ti];
+      // claim the following slot now: its latency hides behind this item
+      if (W.lane == 0) slot0 = atomicAdd(P.task_head, 1u);
       if (W.lane == 0) W.open[W.n_open] = ti;
       W.n_open++;
       cross = T.group == kAllGroups;
       if (cross) {
-        set_cross(P, W, T.cstart, T.cend);
+        if (!cross_mask) set_cross(P, W, T.cstart, T.cend);
+        W.g = kAllGroups;
+        W.gtag = 0;
+        W.cstart = T.cstart;
+        W.cend = T.cend;
+        cross_mask = true;
         if (T.flags & kTaskSeedStart) seed_start_all(P, W);
       } else {
         set_unit(P, W, T.group, T.cstart, T.cend);
+        cross_mask = false;
         if (T.flags & kTaskSeedStart) {
           const GroupMeta gm = P.img.meta[T.group];
           seed_start_of_data(P, W, gm.sod_begin, gm.sod_end);
@@ -719,6 +730,7 @@ NGAP_DEV void close_open(const KParams &P, WaveState &W) {
 // the queue, so a lane looks up its item's group bases when it takes one.
 NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
   TaskStream src;
+  src.start(P, W);
   bool valid = false;
   uint32_t v = 0, p = 0;
   int depth = 0;
