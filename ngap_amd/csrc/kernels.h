@@ -65,13 +65,17 @@ struct KParams {
   const Task *tasks;
   const uint32_t *task_order;  // nullable
   uint32_t num_tasks;
-  uint32_t *task_head;
+  uint32_t *task_head;     // kQueues claim counters, kQueueStride words apart
   unsigned long long *task_items;  // nullable, per task
   uint32_t waves_per_block;
 };
 
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kMaxOpen = 8;  // work items a wave overlaps (k_batched)
+// The work queue is split by XCD: workgroup b runs on XCD b % 8, and queue q
+// holds work items q, q + 8, q + 16, ...; each claim counter has its own
+// 128-byte line.  A wave whose queue is empty takes from the next ones.
+constexpr uint32_t kQueues = 8, kQueueStride = 32;
 constexpr uint32_t kBlockThreads = 256;
 
 // LDS bytes per wave for the given capacities

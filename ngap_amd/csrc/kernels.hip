@@ -659,6 +659,21 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
   }
 }
 
+// Claims the next work item for this wave: from the queue of its XCD first,
+// then from the others in turn.  Returns P.num_tasks when all are empty.
+// `q` is the wave's current queue (advanced when one runs dry).
+NGAP_DEV uint32_t claim_task(const KParams &P, const WaveState &W, uint32_t &q) {
+  for (uint32_t tries = 0; tries < kQueues; tries++) {
+    uint32_t j = 0;
+    if (W.lane == 0) j = atomicAdd(P.task_head + q * kQueueStride, 1u);
+    j = __shfl(j, 0, 64);
+    const uint64_t t = (uint64_t)j * kQueues + q;
+    if (t < P.num_tasks) return (uint32_t)t;
+    q = (q + 1) % kQueues;
+  }
+  return P.num_tasks;
+}
+
 // Batched schedule, engine runs: the wave's stream of work items.  A wave
 // starts the next work item as soon as the current one has no positions left
 // and its queue runs low, so the tail of one work item's items overlaps the
@@ -669,19 +684,15 @@ struct TaskStream {
   uint32_t cur = 0, end = 0;
   bool cross = false;
   bool cross_mask = false;  // LDS holds the cross-group report mask
-  uint32_t slot0 = 0;       // lane 0: the next queue slot, claimed one work item ahead
-  NGAP_DEV void start(const KParams &P, const WaveState &W) {
-    if (W.lane == 0) slot0 = atomicAdd(P.task_head, 1u);
-  }
+  uint32_t q = 0;           // the wave's queue
+  NGAP_DEV void start(const KParams &P, const WaveState &W) { q = blockIdx.x % kQueues; }
   NGAP_DEV int next(const KParams &P, WaveState &W, uint32_t &a, uint32_t &b) {
     if (cur >= end) {
       if (W.n_open >= kMaxOpen) return 1;
-      const uint32_t t = __shfl(slot0, 0, 64);
+      const uint32_t t = claim_task(P, W, q);
       if (t >= P.num_tasks) return 2;
       const uint32_t ti = P.task_order ? P.task_order[t] : t;
       const Task T = P.tasks[ti];
-      // claim the following slot now: its latency hides behind this item
-      if (W.lane == 0) slot0 = atomicAdd(P.task_head, 1u);
       if (W.lane == 0) W.open[W.n_open] = ti;
       W.n_open++;
       cross = T.group == kAllGroups;
@@ -851,10 +862,9 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
     if (W.lane == 0) atomicAdd(P.items, W.items);
     return;
   }
+  uint32_t q = blockIdx.x % kQueues;
   for (;;) {
-    uint32_t t = 0;
-    if (W.lane == 0) t = atomicAdd(P.task_head, 1u);
-    t = __shfl(t, 0, 64);
+    const uint32_t t = claim_task(P, W, q);
     if (t >= P.num_tasks) break;
     const uint32_t ti = P.task_order ? P.task_order[t] : t;
     const Task T = P.tasks[ti];
@@ -1010,7 +1020,7 @@ __global__ void k_reset(unsigned long long *report_count, unsigned long long *it
   if (i == 0) {
     if (!keep_reports) *report_count = 0;
     *items = 0;
-    *task_head = 0;
+    for (uint32_t k = 0; k < kQueues; k++) task_head[k * kQueueStride] = 0;
     *overflow_count = 0;
   }
   for (; i < n_overflow; i += gridDim.x * blockDim.x) overflow[i] = 0;

@@ -59,7 +59,7 @@ int RunCtx::init() {
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
-  HIP_TRY(counters.reserve(64));
+  HIP_TRY(counters.reserve(128 + kQueues * kQueueStride * 4));
   return NGAP_OK;
 }
 
@@ -164,7 +164,7 @@ void attach_ctx(KParams &P, RunCtx &ctx) {
   unsigned long long *counters = ctx.counters.as<unsigned long long>();
   P.report_count = counters + 0;
   P.items = counters + 1;
-  P.task_head = reinterpret_cast<uint32_t *>(counters + 2);
+  P.task_head = reinterpret_cast<uint32_t *>(counters + 16);  // byte 128: the queue counters
   P.overflow_count = reinterpret_cast<uint32_t *>(counters + 3);
   P.reports = ctx.reports.as<uint64_t>();
   P.report_cap = ctx.report_cap;
