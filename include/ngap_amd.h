@@ -107,7 +107,7 @@ typedef struct {
                                    precompute depth is >= 2, 2 = also for one
                                    group, 0 = one group per work item */
   uint32_t ring_capacity;       /* batched: LDS worklist items per wave, a power
-                                   of two in [64, 2048], 0 = 512 */
+                                   of two in [64, 2048], 0 = 256 */
   int32_t task_traces;          /* batched: record the items of every work item
                                    (ngap_get_task_items); a wave then finishes
                                    one work item before it starts the next */

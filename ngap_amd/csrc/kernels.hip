@@ -359,11 +359,15 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
   const uint32_t b1 = k >= 2 ? in[p + 1] : 0u;
   const uint32_t b2 = k >= 3 ? in[p + 2] : 0u;
   const uint32_t kReportOnly = 1u << 31;
+  // depth-2 lists: the deepest level at depth 2; at depth 3 only where two
+  // bytes are left or the 2-gram has reporting depth-2 lists
+  const uint32_t key2 = (b0 << 8) | b1;
+  const bool need2 = k >= 2 && (P.memo_depth == 2 || k == 2 ||
+                                ((P.img.x2rep[key2 >> 5] >> (key2 & 31)) & 1u));
   uint32_t e0 = 0, ec = 0;
-  if (k >= 2) {
-    const uint32_t key = (b0 << 8) | b1;
-    e0 = P.img.xr2[key];
-    ec = P.img.xr2[key + 1] - e0;
+  if (need2) {
+    e0 = P.img.xr2[key2];
+    ec = P.img.xr2[key2 + 1] - e0;
   }
   {  // depth 1: reporting entries only (positions that continue use depth >= 2)
     uint32_t r0 = 0, rc = 0;
@@ -389,8 +393,6 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
     // positions with three bytes left: depth-3 lists of every group from the
     // 3-gram index, plus depth-2 reports where the 2-gram has any; the two
     // positions before the chunk end use depth 2 and 1 as above
-    const bool rep2 = k >= 3 && ((P.img.x2rep[((b0 << 8) | b1) >> 5] >> (b1 & 31)) & 1u);
-    if (k != 2 && !rep2) ec = 0;
     if (wballot(ec != 0)) {
       for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
         const uint32_t pos = __shfl(p, (int)o, 64);
@@ -524,6 +526,9 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int32_t e = es[s];
+      // successors fill the slots from 0 (a continuation link needs slots
+      // 0-2 full): once no lane has one in this slot, none has one later
+      if (s < 3 && wballot(more && e >= 0) == 0) break;
       const uint32_t u = (uint32_t)e & (uint32_t)kIdMask;
       const bool hit = test_hit(more, e, ts[s], cw[s], b);
       const uint64_t it = item_of(u | lg.gtag, p1);
@@ -854,14 +859,6 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
   WaveState W;
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
-  if (P.chunk_len && !P.task_items) {
-    // engine runs without per-work-item traces: overlapping work items
-    clear_dedup(P, W);
-    drain_stream(P, W);
-    flush_reports(P, W);
-    if (W.lane == 0) atomicAdd(P.items, W.items);
-    return;
-  }
   uint32_t q = blockIdx.x % kQueues;
   for (;;) {
     const uint32_t t = claim_task(P, W, q);
@@ -891,6 +888,22 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
       if (P.task_items) P.task_items[ti] = W.items - items0;
     }
   }
+  flush_reports(P, W);
+  if (W.lane == 0) atomicAdd(P.items, W.items);
+}
+
+// The batched engine for engine runs without per-work-item traces (chunks
+// of chunk_len bytes): each wave overlaps its work items (drain_stream).  A
+// kernel of its own, so its registers are not sized for the per-item path.
+__global__ void __launch_bounds__(kBlockThreads) k_batched_stream(KParams P) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t wbytes = wave_lds_bytes(P.ring_cap, P.rbuf_cap, P.dc_cap);
+  WaveState W;
+  setup_wave(P, W, lds + 16 + wave * wbytes,
+             P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
+  clear_dedup(P, W);
+  drain_stream(P, W);
   flush_reports(P, W);
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
@@ -1049,13 +1062,21 @@ hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint
 }
 
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds) {
-  hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+  if (P.chunk_len && !P.task_items)
+    hipLaunchKernelGGL(k_batched_stream, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+  else
+    hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);
   return hipGetLastError();
 }
 
 hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batched,
-                                                      kBlockThreads, lds);
+  // the grid is sized for the kernel with the lower occupancy of the two
+  int a = 0, b = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batched, kBlockThreads, lds);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batched_stream, kBlockThreads, lds);
+  *blocks_per_cu = a < b ? a : b;
+  return e;
 }
 
 hipError_t set_lds_limit(uint32_t lds) {
@@ -1063,6 +1084,9 @@ hipError_t set_lds_limit(uint32_t lds) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_blocking),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched_stream),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),

@@ -740,7 +740,7 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
   int rc = timeit(cur, best_t);
   if (rc) return rc;
   const std::vector<uint32_t> windows = {256, 512, 1024, 2048, 4096, 8192, 16384};
-  const std::vector<uint32_t> rings = {128, 256, 1024};
+  const std::vector<uint32_t> rings = {128, 512, 1024};
   const std::vector<uint32_t> spills = {8192, 16384, 262144};
   for (int axis = 0; axis < 4; axis++) {
     // the grid axis scales the full grid of the ring capacity picked so far

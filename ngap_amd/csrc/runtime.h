@@ -25,7 +25,7 @@ int ngap_fail(int code, const std::string &msg);
 namespace ngapamd {
 
 // LDS capacities per wave (entries)
-constexpr uint32_t kRing = 512, kRbuf = 128, kDc = 256;
+constexpr uint32_t kRing = 256, kRbuf = 128, kDc = 256;
 constexpr uint32_t kMinRing = 64, kMaxRing = 2048;  // batched ring_capacity range
 
 struct DevBuf {
