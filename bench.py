@@ -125,7 +125,7 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
-def learned_schedule(eng, groups, train, test, chunk, depth, waves=4096):
+def learned_schedule(eng, groups, train, test, chunk, depth):
     """Trains the frontier-size MLP on traces of `train` (at most 10^6
     (group, 4 KiB window) samples) and shapes the batched schedule of `test`
     from its predictions.  Leaves `test` loaded.  Returns (options, info)."""
@@ -151,6 +151,7 @@ def learned_schedule(eng, groups, train, test, chunk, depth, waves=4096):
                          -1).reshape(-1, 2).astype(np.uint32)
     hist = learned.window_hist(test, chunk)
     pred = m.predict_pairs(hist, stats, all_pairs)
+    waves = eng.batched_waves()
     cost = learned.predicted_cost(pred).reshape(len(groups), S * K)
     if depth >= 2 and len(groups) > 1:   # all-group work items
         tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, waves)
@@ -163,7 +164,7 @@ def learned_schedule(eng, groups, train, test, chunk, depth, waves=4096):
     tpred = m.predict_pairs(hist, stats, tp)
     info = {"train_traces": int(x.shape[0]), "train_loss": round(float(loss), 5),
             "train_s": round(t_train, 2), "schedule_host_s": round(t_sched, 3),
-            "tasks": int(len(tasks)), "spill_capacity": int(spill),
+            "tasks": int(len(tasks)), "spill_capacity": int(spill), "waves": int(waves),
             "mae_log_items": round(float(np.mean(np.abs(tpred - ty))), 4),
             "mae_constant_predictor": round(float(np.mean(np.abs(np.mean(y) - ty))), 4)}
     m.close()

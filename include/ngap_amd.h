@@ -152,6 +152,13 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats)
 int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
                   ngap_run_opts *best, double *best_ms);
 
+/* The batched engine's persistent grid for a ring capacity (0 = default):
+ * workgroups and waves per workgroup, for callers that shape their own work
+ * items (about two per wave keep every wave busy).  No counterpart in the
+ * reference (its grid is (groups, chunks)). */
+int ngap_batched_grid(ngap_engine *eng, uint32_t ring_capacity, uint32_t *blocks,
+                      uint32_t *waves_per_block);
+
 /* Sorted, de-duplicated reports of the last run, reference encoding
  * ((state | group << 22) << 32) | offset.  *n_out gets the total count; at
  * most cap entries are written (out may be NULL to query the count). */

@@ -98,6 +98,8 @@ SIGNATURES = {
     "ngap_run": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     "ngap_autotune": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.c_int32, C.POINTER(RunOpts),
                                 C.POINTER(C.c_double)]),
+    "ngap_batched_grid": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32)]),
     "ngap_get_reports": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "ngap_get_task_items": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "ngap_get_tasks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -343,10 +343,20 @@ namespace {
 // Static schedule (no host work items): a group whose roots are rare under the
 // input's byte histogram takes whole chunks per work item; others take
 // `window` positions.  Work items run longest-expected-first.
-std::vector<Task> static_schedule(const ngap_engine *eng, uint32_t window, bool is_dup) {
+// Work-item length near `bytes` / (2 x waves), rounded to a power of two
+// (256 .. 65536): about two work items per wave.
+uint32_t auto_window(uint64_t bytes, uint32_t waves) {
+  const double per = (double)bytes / std::max<uint64_t>(1, 2ull * waves);
+  uint32_t win = 256;
+  while (win < 65536 && (double)win * 1.41421356 < per) win *= 2;
+  return win;
+}
+
+std::vector<Task> static_schedule(const ngap_engine *eng, uint32_t window, bool is_dup,
+                                  uint32_t waves) {
   const Image &im = eng->img;
   const uint32_t G = (uint32_t)im.num_groups, S = eng->num_seg, L = (uint32_t)eng->chunk_len;
-  uint32_t win = window ? window : 4096;
+  uint32_t win = window ? window : auto_window((uint64_t)G * S * L, waves);
   if (win > L) win = L;
   std::vector<double> root_p(G, 0.0);
   for (uint32_t g = 0; g < G; g++)
@@ -378,12 +388,7 @@ std::vector<Task> static_schedule(const ngap_engine *eng, uint32_t window, bool 
 std::vector<Task> cross_schedule(const ngap_engine *eng, uint32_t window, bool is_dup,
                                  uint32_t waves) {
   const uint32_t S = eng->num_seg, L = (uint32_t)eng->chunk_len;
-  uint32_t win = window;
-  if (!win) {
-    const uint64_t per = (uint64_t)S * L / std::max<uint64_t>(1, 4ull * waves);
-    win = 256;
-    while ((uint64_t)win * 2 <= per && win < 65536) win *= 2;
-  }
+  uint32_t win = window ? window : auto_window((uint64_t)S * L, waves);
   if (win > L) win = L;
   std::vector<Task> tasks;
   for (uint32_t c = 0; c < S; c++)
@@ -675,7 +680,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
                                              (o.cross_groups == 1 && eng->img.num_groups > 1)))
                        ? 1 : 2;
   uint32_t waves = 0;
-  if (kind == 1) {
+  if (kind != 0) {
     uint32_t blocks = 0;
     int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap);
     if (rc) return rc;
@@ -698,7 +703,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
     } else if (kind == 1) {
       sc.tasks = cross_schedule(eng, o.window, o.is_dup != 0, waves);
     } else {
-      sc.tasks = static_schedule(eng, o.window, o.is_dup != 0);
+      sc.tasks = static_schedule(eng, o.window, o.is_dup != 0, waves);
     }
     sc.generation = eng->generation;
     sc.kind = kind;
@@ -781,6 +786,21 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
   }
   *best = cur;
   if (best_ms) *best_ms = best_t;
+  return NGAP_OK;
+}
+
+int ngap_batched_grid(ngap_engine *eng, uint32_t ring_capacity, uint32_t *blocks,
+                      uint32_t *waves_per_block) {
+  if (!eng || !blocks || !waves_per_block) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  ngap_run_opts o;
+  ngap_default_opts(&o);
+  o.ring_capacity = ring_capacity;
+  const uint32_t ring = ring_of(o);
+  if (!ring) return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
+  HIP_TRY(hipSetDevice(eng->device));
+  int rc = persistent_blocks(eng, 0, blocks, ring);
+  if (rc) return rc;
+  *waves_per_block = kBlockThreads / kWave;
   return NGAP_OK;
 }
 

@@ -95,6 +95,12 @@ class Engine:
             o._keep = arr  # keep the array alive with the options
         return o
 
+    def batched_waves(self, ring_capacity=0):
+        """Waves of the batched engine's persistent grid."""
+        b, w = C.c_uint32(), C.c_uint32()
+        check(_lib.lib().ngap_batched_grid(self._h, int(ring_capacity), C.byref(b), C.byref(w)))
+        return b.value * w.value
+
     def autotune(self, max_runs=16, **kw):
         """Batched-engine knobs (window, persistent blocks, spill capacity)
         timed on the loaded input; returns (options, best kernel ms)."""

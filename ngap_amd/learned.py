@@ -100,7 +100,7 @@ def features(hist, stats, pairs):
 
 
 def schedule_from_costs(cost, num_groups, num_seg, chunk, waves, window=WINDOW,
-                        per_wave=4):
+                        per_wave=2):
     """Merges consecutive windows of each (group, chunk) while the summed cost
     stays under total / (waves * per_wave); returns work items ordered by
     decreasing cost and their costs.  cost: [num_seg * K * ...] indexed by
@@ -126,7 +126,7 @@ def schedule_from_costs(cost, num_groups, num_seg, chunk, waves, window=WINDOW,
 ALL_GROUPS = 0xFFFFFFFF
 
 
-def cross_schedule_from_costs(cost, num_seg, chunk, waves, window=WINDOW, per_wave=4,
+def cross_schedule_from_costs(cost, num_seg, chunk, waves, window=WINDOW, per_wave=2,
                               min_piece=256):
     """All-group work items (every group's roots in one window, precompute
     depth >= 2) shaped by per-(group, window) costs: a window's cost is the

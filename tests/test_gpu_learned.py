@@ -89,7 +89,9 @@ def test_learned_schedule_end_to_end(engine):
     S = len(test) // chunk
     cost = np.zeros((len(groups), S * learned.windows_per_chunk(chunk)))
     cost[tp[:, 0], tp[:, 1]] = learned.predicted_cost(pred)
-    tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves=1024)
+    waves = engine.batched_waves()
+    assert waves > 0
+    tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves=waves)
     want, _ = oracle.run(groups, test, chunk)
     say("learned schedule run")
     st = engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
@@ -97,7 +99,7 @@ def test_learned_schedule_end_to_end(engine):
                     fallback=True)
     assert np.array_equal(engine.reports(), want)
     assert st["num_tasks"] == len(tasks)
-    ctasks, ccost = learned.cross_schedule_from_costs(cost, S, chunk, waves=1024)
+    ctasks, ccost = learned.cross_schedule_from_costs(cost, S, chunk, waves=waves)
     say("learned all-group schedule run")
     st = engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
                     unique=True, tasks=ctasks, spill_capacity=learned.spill_capacity(pred, ccost),
