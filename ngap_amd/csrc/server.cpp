@@ -78,6 +78,11 @@ struct Request {
 struct Slot {
   RunCtx ctx;
   DevBuf input;
+  Task *task_host = nullptr;    // pinned staging of the work items (async upload)
+  size_t task_cap = 0;
+  ~Slot() {
+    if (task_host) (void)hipHostFree(task_host);
+  }
   bool busy = false;
   std::vector<uint64_t> ids;
   std::vector<uint32_t> base;   // request start offsets in the slot input
@@ -87,7 +92,7 @@ struct Slot {
   uint32_t blocks = 0;
 };
 
-constexpr uint32_t kServeWindow = 4096;
+constexpr uint32_t kMaxServeWindow = 4096;  // work-item positions, at most
 constexpr uint32_t kServeSpill = 16384;  // worklist spill items per wave
 constexpr uint64_t kMinLaunchBytes = 1 << 20;
 
@@ -138,13 +143,19 @@ int launch_slot(ngap_server *s, Slot &sl) {
   }
   if (sl.ids.empty()) return NGAP_OK;
   HIP_TRY(sl.input.reserve(total + 16));
+  // work items of about total / (2 x waves) positions (a power of two in
+  // 256 .. 4096): a short request spreads over many waves, which is what
+  // bounds its latency
+  const uint64_t waves = (uint64_t)s->blocks_per_launch * (kBlockThreads / kWave);
+  uint32_t win = 256;
+  while (win < kMaxServeWindow && (uint64_t)win * 2 * waves < total) win *= 2;
   for (size_t i = 0; i < sl.ids.size(); i++) {
     Request &r = s->reqs[sl.ids[i]];
     HIP_TRY(hipMemcpyAsync(sl.input.as<uint8_t>() + sl.base[i], r.data, r.len,
                            hipMemcpyHostToDevice, sl.ctx.stream));
     const uint32_t cs = sl.base[i], ce = cs + (uint32_t)r.len;
-    for (uint32_t a = cs; a < ce; a += kServeWindow) {
-      const uint32_t b = std::min<uint32_t>(a + kServeWindow, ce);
+    for (uint32_t a = cs; a < ce; a += win) {
+      const uint32_t b = std::min<uint32_t>(a + win, ce);
       sl.tasks.push_back({(uint32_t)r.group, cs, ce, a, b, a == cs ? kTaskSeedStart : 0u});
     }
     r.state = 1;
@@ -154,7 +165,18 @@ int launch_slot(ngap_server *s, Slot &sl) {
     return (x.cend - x.cstart) > (y.cend - y.cstart);
   });
   const uint64_t report_cap = std::max<uint64_t>(1 << 20, total * 2);
-  HIP_TRY(sl.ctx.tasks.upload(sl.tasks));
+  if (sl.task_cap < sl.tasks.size()) {
+    if (sl.task_host) HIP_TRY(hipHostFree(sl.task_host));
+    sl.task_host = nullptr;
+    sl.task_cap = 0;
+    const size_t cap = std::max<size_t>(4096, sl.tasks.size() * 2);
+    HIP_TRY(hipHostMalloc((void **)&sl.task_host, cap * sizeof(Task), hipHostMallocDefault));
+    sl.task_cap = cap;
+  }
+  std::copy(sl.tasks.begin(), sl.tasks.end(), sl.task_host);
+  HIP_TRY(sl.ctx.tasks.reserve(sl.tasks.size() * sizeof(Task)));
+  HIP_TRY(hipMemcpyAsync(sl.ctx.tasks.p, sl.task_host, sl.tasks.size() * sizeof(Task),
+                         hipMemcpyHostToDevice, sl.ctx.stream));
   sl.ctx.tasks_resident = false;
   HIP_TRY(sl.ctx.overflow.reserve(sl.tasks.size() * 4 + 4));
   HIP_TRY(sl.ctx.reports.reserve(report_cap * 8));
