@@ -349,34 +349,81 @@ NGAP_DEV void for_entries(const WaveState &W, uint32_t e0, uint32_t ecnt, F &&f)
 // groups whose all-input states report on it, its 2-gram the depth-2 lists of
 // every group, each continued at depth 3 through that list's rank table.
 // The same lists as roots() visits group by group, so the same items.
-NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
+// Per-lane state of one position in roots_cross.
+struct CrossLane {
+  uint32_t p, k, b0, e0, ec, t0, tc, r0, rc;
+};
+
+// The loads of one position's lookups, in stages shared by both halves of a
+// 128-position batch: input bytes, then the index offsets.
+NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, uint32_t pend,
+                           CrossLane (&L)[2]) {
   const uint8_t *in = P.input;
-  const uint32_t p = p0 + W.lane;
-  const bool act = p < pend;
-  const uint32_t left = act ? W.cend - p : 0u;
-  const uint32_t k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
-  const uint32_t b0 = act ? in[p] : 0u;
-  const uint32_t b1 = k >= 2 ? in[p + 1] : 0u;
-  const uint32_t b2 = k >= 3 ? in[p + 2] : 0u;
-  const uint32_t kReportOnly = 1u << 31;
-  // depth-2 lists: the deepest level at depth 2; at depth 3 only where two
-  // bytes are left or the 2-gram has reporting depth-2 lists
-  const uint32_t key2 = (b0 << 8) | b1;
-  const bool need2 = k >= 2 && (P.memo_depth == 2 || k == 2 ||
-                                ((P.img.x2rep[key2 >> 5] >> (key2 & 31)) & 1u));
-  uint32_t e0 = 0, ec = 0;
-  if (need2) {
-    e0 = P.img.xr2[key2];
-    ec = P.img.xr2[key2 + 1] - e0;
+  uint32_t b1[2], b2[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t p = p0 + h * kWave + W.lane;
+    const bool act = p < pend;
+    const uint32_t left = act ? W.cend - p : 0u;
+    L[h].p = p;
+    L[h].k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
+    L[h].b0 = act ? in[p] : 0u;
+    b1[h] = L[h].k >= 2 ? in[p + 1] : 0u;
+    b2[h] = L[h].k >= 3 ? in[p + 2] : 0u;
   }
-  {  // depth 1: reporting entries only (positions that continue use depth >= 2)
-    uint32_t r0 = 0, rc = 0;
-    if (act && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u)) {
-      r0 = P.img.xr1[b0];
-      rc = P.img.xr1[b0 + 1] - r0;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t k = L[h].k, b0 = L[h].b0;
+    // depth-2 lists: the deepest level at depth 2; at depth 3 only where two
+    // bytes are left or the 2-gram has reporting depth-2 lists
+    const uint32_t key2 = (b0 << 8) | b1[h];
+    const bool need2 = k >= 2 && (P.memo_depth == 2 || k == 2 ||
+                                  ((P.img.x2rep[key2 >> 5] >> (key2 & 31)) & 1u));
+    L[h].e0 = L[h].ec = 0;
+    if (need2) {
+      L[h].e0 = P.img.xr2[key2];
+      L[h].ec = P.img.xr2[key2 + 1] - L[h].e0;
     }
-    if (wballot(rc != 0)) {
-      for_entries(W, r0, rc, [&](bool ok, uint32_t idx, uint32_t o) {
+    L[h].t0 = L[h].tc = 0;
+    if (P.memo_depth >= 3 && k >= 3) {
+      const uint32_t key3 = (key2 << 8) | b2[h];
+      L[h].t0 = P.img.xr3[key3];
+      L[h].tc = P.img.xr3[key3 + 1] - L[h].t0;
+    }
+    L[h].r0 = L[h].rc = 0;
+    if (k >= 1 && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u)) {
+      L[h].r0 = P.img.xr1[b0];
+      L[h].rc = P.img.xr1[b0 + 1] - L[h].r0;
+    }
+  }
+}
+
+// One entry per lane at most: the entry and its single list item are loaded
+// for both halves before either is emitted.
+NGAP_DEV void emit_one(const KParams &P, WaveState &W, bool ok, int32_t e, bool ronly,
+                       uint32_t pos, uint32_t tag) {
+  const uint64_t it = item_of(((uint32_t)e & (uint32_t)kIdMask) | tag, pos);
+  emit(P, W, ok && (e & kFlagReport), it);
+  bool keep = ok && !ronly && !(e & kFlagDeadEnd) && (pos + 1 < W.cend);
+  if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
+  push(P, W, keep, it);
+}
+
+// Roots of every group for positions [p0, pend), pend - p0 <= 128 (cross-
+// group work items, memo depth >= 2): one lane per position and half; the
+// position's byte selects the groups whose all-input states report on it,
+// its 2-gram the depth-2 lists and its 3-gram the depth-3 lists of every
+// group.  The same lists as roots() visits group by group, so the same items.
+NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
+  const uint32_t kReportOnly = 1u << 31;
+  CrossLane L[2];
+  cross_lookup(P, W, p0, pend, L);
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t p = L[h].p;
+    // depth 1: reporting entries only (positions that continue use depth >= 2)
+    if (wballot(L[h].rc != 0)) {
+      for_entries(W, L[h].r0, L[h].rc, [&](bool ok, uint32_t idx, uint32_t o) {
         const uint32_t pos = __shfl(p, (int)o, 64);
         uint32_t beg = 0, cnt = 0, tag = 0;
         if (ok) {
@@ -388,37 +435,59 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
         flatten(P, W, cnt, beg, pos, tag, P.img.list[0]);
       });
     }
-  }
-  if (P.memo_depth >= 3) {
-    // positions with three bytes left: depth-3 lists of every group from the
-    // 3-gram index, plus depth-2 reports where the 2-gram has any; the two
-    // positions before the chunk end use depth 2 and 1 as above
-    if (wballot(ec != 0)) {
-      for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
+    // depth 2: full lists where it is the deepest level, reports otherwise
+    if (wballot(L[h].ec != 0)) {
+      const uint32_t k = L[h].k;
+      for_entries(W, L[h].e0, L[h].ec, [&](bool ok, uint32_t idx, uint32_t o) {
         const uint32_t pos = __shfl(p, (int)o, 64);
         const uint32_t ko = __shfl(k, (int)o, 64);
         uint32_t beg = 0, cnt = 0, tag = 0;
         if (ok) {
+          // (list index, begin, end, reporting entries | group << 22)
           const uint4 e = reinterpret_cast<const uint4 *>(P.img.xe2)[idx];
           tag = (e.w >> 22) << 22;
           beg = e.y;
-          cnt = ko == 2 ? e.z - e.y : (e.w & (uint32_t)kIdMask);
-          if (ko != 2) beg |= kReportOnly;
+          cnt = ko == 2 || P.memo_depth == 2 ? e.z - e.y : (e.w & (uint32_t)kIdMask);
+          if (!(ko == 2 || P.memo_depth == 2)) beg |= kReportOnly;
         }
         flatten(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
       });
     }
-    uint32_t t0 = 0, tc = 0;
-    if (k >= 3) {
-      const uint32_t key3 = (b0 << 16) | (b1 << 8) | b2;
-      t0 = P.img.xr3[key3];
-      tc = P.img.xr3[key3 + 1] - t0;
+  }
+  if (P.memo_depth < 3) return;
+  // depth 3 from the 3-gram index: (begin in xl3, length | group << 22)
+  const bool any0 = wballot(L[0].tc != 0) != 0, any1 = wballot(L[1].tc != 0) != 0;
+  if (!any0 && !any1) return;
+  if (wballot(L[0].tc > 1 || L[1].tc > 1) == 0) {
+    // the common case: at most one list per lane and half, loaded together
+    uint2 e[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+      e[h] = L[h].tc ? reinterpret_cast<const uint2 *>(P.img.xe3)[L[h].t0] : make_uint2(0, 0);
+    if (wballot((e[0].y & (uint32_t)kIdMask) > 1 || (e[1].y & (uint32_t)kIdMask) > 1) == 0) {
+      int32_t x[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) x[h] = (e[h].y & (uint32_t)kIdMask) ? P.img.xl3[e[h].x] : 0;
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        emit_one(P, W, (e[h].y & (uint32_t)kIdMask) != 0, x[h], false, L[h].p + 2,
+                 (e[h].y >> 22) << 22);
+      return;
     }
-    if (wballot(tc != 0) == 0) return;
-    for_entries(W, t0, tc, [&](bool ok, uint32_t idx, uint32_t o) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+      flatten(P, W, e[h].y & (uint32_t)kIdMask, e[h].x, L[h].p + 2, (e[h].y >> 22) << 22,
+              P.img.xl3);
+    return;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t p = L[h].p;
+    if (wballot(L[h].tc != 0) == 0) continue;
+    for_entries(W, L[h].t0, L[h].tc, [&](bool ok, uint32_t idx, uint32_t o) {
       const uint32_t pos = __shfl(p, (int)o, 64);
       uint32_t beg = 0, cnt = 0, tag = 0;
-      if (ok) {  // (begin in xl3, length | group << 22)
+      if (ok) {
         const uint2 e = reinterpret_cast<const uint2 *>(P.img.xe3)[idx];
         beg = e.x;
         cnt = e.y & (uint32_t)kIdMask;
@@ -426,21 +495,7 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
       }
       flatten(P, W, cnt, beg, pos + 2, tag, P.img.xl3);
     });
-    return;
   }
-  if (wballot(ec != 0) == 0) return;
-  for_entries(W, e0, ec, [&](bool ok, uint32_t idx, uint32_t o) {
-    const uint32_t pos = __shfl(p, (int)o, 64);
-    uint32_t beg = 0, cnt = 0, tag = 0;
-    if (ok) {
-      // (list index, begin, end, reporting entries | group << 22)
-      const uint4 e = reinterpret_cast<const uint4 *>(P.img.xe2)[idx];
-      tag = (e.w >> 22) << 22;
-      beg = e.y;
-      cnt = e.z - e.y;
-    }
-    flatten(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
-  });
 }
 
 NGAP_DEV void seed_start_of_data(const KParams &P, WaveState &W, uint32_t sod_begin,
@@ -721,7 +776,8 @@ struct TaskStream {
       end = T.end;
     }
     a = cur;
-    b = cur + kWave < end ? cur + kWave : end;
+    const uint32_t step = cross ? 2 * kWave : kWave;  // roots_cross takes two halves
+    b = cur + step < end ? cur + step : end;
     cur = b;
     return 0;
   }
