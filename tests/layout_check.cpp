@@ -108,6 +108,7 @@ int main() {
     const int ng = 1 + trial * 3;
     std::vector<G> gs;
     for (int i = 0; i < ng; i++) gs.push_back(random_group(r, 5 + r() % 60));
+    if (ng > 1) gs[1] = random_group(r, 0);  // a group without states
     std::vector<HostGroup> hg(ng);
     for (int i = 0; i < ng; i++) {
       hg[i].num_states = gs[i].n;

@@ -223,3 +223,35 @@ def test_reports_bit_exact_after_reload(engine):
         data = synth.zipf_bytes(rng, 1 << 15, 1.1)
         check_all(engine, [g], data, 1 << 13, None,
                   [dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2)])
+
+
+@pytest.mark.parametrize("is_dup", [False, True])
+def test_empty_sod_only_and_ragged_input(engine, is_dup):
+    # a group without states, a group with start-of-data states only, an
+    # all-input group and a regex group; the input's tail beyond the last
+    # whole chunk is dropped by both the engine and the oracle
+    r = random.Random(77)
+    alpha = list(b"abc")
+    sod_g, _ = H.chains_group([H.random_spec(r, 3, alpha) for _ in range(5)], start="start-of-data")
+    aas_g, _ = H.chains_group([H.random_spec(r, 4, alpha) for _ in range(8)])
+    empty = make_group(0, [], [])
+    regex = synth.regex_group(np.random.default_rng(5), 400)
+    data = bytes(r.choice(alpha) for _ in range(1000)) + b"xyz"
+    cfgs = [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d, cross_groups=x)
+            for s, d, x in (("plain", 1, 1), ("plain", 3, 1), ("batched", 1, 1),
+                            ("batched", 2, 0), ("batched", 2, 1), ("batched", 3, 1))]
+    cfgs.append(dict(algorithm="blockinggroups", schedule="plain"))
+    check_all(engine, [empty, sod_g, aas_g, regex, empty], data, 100, None, cfgs, is_dup=is_dup)
+
+
+def test_a_thousand_and_twenty_four_groups(engine):
+    # the group-id budget of the report encoding (10 bits) used in full
+    r = random.Random(1024)
+    alpha = list(b"acgt")
+    groups = [H.chains_group([H.random_spec(r, r.randint(1, 4), alpha)
+                              for _ in range(r.randint(1, 3))])[0] for _ in range(1024)]
+    data = bytes(r.choice(alpha) for _ in range(1 << 13))
+    cfgs = [dict(algorithm="nonblockingallgroups", schedule=s, precompute_depth=d, unique=u)
+            for s, d, u in (("plain", 2, 1), ("batched", 2, 0), ("batched", 3, 1))]
+    want = check_all(engine, groups, data, 1 << 11, None, cfgs)
+    assert int(np.max(want >> np.uint64(54))) == 1023 or want.size == 0
