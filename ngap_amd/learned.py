@@ -99,30 +99,6 @@ def features(hist, stats, pairs):
     return np.concatenate([hist[pairs[:, 1]], stats[pairs[:, 0]]], 1).astype(np.float32)
 
 
-def schedule_from_costs(cost, num_groups, num_seg, chunk, waves, window=WINDOW,
-                        per_wave=2):
-    """Merges consecutive windows of each (group, chunk) while the summed cost
-    stays under total / (waves * per_wave); returns work items ordered by
-    decreasing cost and their costs.  cost: [num_seg * K * ...] indexed by
-    (group, global window) as cost[g, w]."""
-    K = windows_per_chunk(chunk, window)
-    cost = np.asarray(cost, np.float64).reshape(num_groups, num_seg * K)
-    target = max(cost.sum() / max(1, waves * per_wave), 1e-9)
-    tasks, tcost = [], []
-    for g in range(num_groups):
-        for c in range(num_seg):
-            row = cost[g, c * K:(c + 1) * K]
-            a, acc = 0, 0.0
-            for k in range(K):
-                acc += row[k]
-                if acc >= target or k == K - 1:
-                    tasks.append((g, c, a * window, min((k + 1) * window, chunk)))
-                    tcost.append(acc)
-                    a, acc = k + 1, 0.0
-    order = np.argsort(-np.asarray(tcost), kind="stable")
-    return np.asarray(tasks, np.uint32)[order], np.asarray(tcost)[order]
-
-
 ALL_GROUPS = 0xFFFFFFFF
 
 
@@ -130,37 +106,62 @@ def cross_schedule_from_costs(cost, num_seg, chunk, waves, window=WINDOW, per_wa
                               min_piece=256):
     """All-group work items (every group's roots in one window, precompute
     depth >= 2) shaped by per-(group, window) costs: a window's cost is the
-    sum over groups; a window above the target total / (waves * per_wave) is
-    cut into equal pieces (at least min_piece bytes), consecutive windows
-    below it are merged.  Returns work items ordered by decreasing cost and
+    sum over groups, then windows are cut or merged as in
+    schedule_from_costs.  Returns work items ordered by decreasing cost and
     their costs."""
     K = windows_per_chunk(chunk, window)
     wcost = np.asarray(cost, np.float64).reshape(-1, num_seg * K).sum(0)
     target = max(wcost.sum() / max(1, waves * per_wave), 1e-9)
     tasks, tcost = [], []
     for c in range(num_seg):
-        a, acc = 0, 0.0
-        for k in range(K):
-            lo, hi = k * window, min((k + 1) * window, chunk)
-            w = wcost[c * K + k]
-            if w >= target:
-                if acc > 0:
-                    tasks.append((ALL_GROUPS, c, a, lo))
-                    tcost.append(acc)
-                n = int(min(np.ceil(w / target), max(1, (hi - lo) // min_piece)))
-                cuts = [lo + (hi - lo) * i // n for i in range(n + 1)]
-                for x, y in zip(cuts, cuts[1:]):
-                    tasks.append((ALL_GROUPS, c, x, y))
-                    tcost.append(w / n)
-                a, acc = hi, 0.0
-                continue
-            acc += w
-            if acc >= target or k == K - 1:
-                tasks.append((ALL_GROUPS, c, a, hi))
-                tcost.append(acc)
-                a, acc = hi, 0.0
+        _tile(tasks, tcost, ALL_GROUPS, c, wcost[c * K:(c + 1) * K], chunk, window, target,
+              min_piece)
     order = np.argsort(-np.asarray(tcost), kind="stable")
     return np.asarray(tasks, np.uint32)[order], np.asarray(tcost)[order]
+
+
+def schedule_from_costs(cost, num_groups, num_seg, chunk, waves, window=WINDOW,
+                        per_wave=2, min_piece=256):
+    """Per-group work items shaped by per-(group, window) costs: inside each
+    (group, chunk), a window above the target total / (waves * per_wave) is cut
+    into equal pieces (at least min_piece bytes) and consecutive windows below
+    it are merged.  Returns work items ordered by decreasing cost and their
+    costs.  cost: [num_groups, num_seg * K] indexed by (group, global window)."""
+    K = windows_per_chunk(chunk, window)
+    cost = np.asarray(cost, np.float64).reshape(num_groups, num_seg * K)
+    target = max(cost.sum() / max(1, waves * per_wave), 1e-9)
+    tasks, tcost = [], []
+    for g in range(num_groups):
+        for c in range(num_seg):
+            _tile(tasks, tcost, g, c, cost[g, c * K:(c + 1) * K], chunk, window, target,
+                  min_piece)
+    order = np.argsort(-np.asarray(tcost), kind="stable")
+    return np.asarray(tasks, np.uint32)[order], np.asarray(tcost)[order]
+
+
+def _tile(tasks, tcost, g, c, row, chunk, window, target, min_piece):
+    """Work items of one (group or all groups, chunk) from its window costs."""
+    a, acc = 0, 0.0
+    K = len(row)
+    for k in range(K):
+        lo, hi = k * window, min((k + 1) * window, chunk)
+        w = row[k]
+        if w >= target:
+            if acc > 0:
+                tasks.append((g, c, a, lo))
+                tcost.append(acc)
+            n = int(min(np.ceil(w / target), max(1, (hi - lo) // min_piece)))
+            cuts = [lo + (hi - lo) * i // n for i in range(n + 1)]
+            for x, y in zip(cuts, cuts[1:]):
+                tasks.append((g, c, x, y))
+                tcost.append(w / n)
+            a, acc = hi, 0.0
+            continue
+        acc += w
+        if acc >= target or k == K - 1:
+            tasks.append((g, c, a, hi))
+            tcost.append(acc)
+            a, acc = hi, 0.0
 
 
 def predicted_cost(pred_log_items, window=WINDOW):
