@@ -19,6 +19,13 @@ uint64_t Image::bytes() const {
 
 namespace {
 
+// Calls f(b) for every byte b of state v's class, in increasing order.
+template <class F>
+inline void for_each_byte(const uint32_t *sets, int v, F &&f) {
+  for (int w = 0; w < 8; w++)
+    for (uint32_t x = sets[(size_t)v * 8 + w]; x; x &= x - 1) f(w * 32 + __builtin_ctz(x));
+}
+
 inline bool has_byte(const uint32_t *sets, int v, int b) {
   return (sets[(size_t)v * 8 + (b >> 5)] >> (b & 31)) & 1u;
 }
@@ -176,8 +183,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     // memo depth 1: all-input states by byte (all 256 keys present)
     std::vector<std::vector<int32_t>> l1(256);
     for (int i = 0; i < hg.num_aas; i++)
-      for (int b = 0; b < 256; b++)
-        if (has_byte(hg.sets, hg.aas[i], b)) l1[b].push_back(hg.aas[i]);
+      for_each_byte(hg.sets, hg.aas[i], [&](int b) { l1[b].push_back(hg.aas[i]); });
     size_t m1 = img.mask1.size();
     img.mask1.resize(m1 + 16, 0u);
     for (int b = 0; b < 256; b++) {
@@ -209,9 +215,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
             if (!mark[u]) { mark[u] = 1; S.push_back(u); }
         for (int32_t u : S) mark[u] = 0;
         for (auto &bk : bucket) bk.clear();
-        for (int32_t u : S)
-          for (int b = 0; b < 256; b++)
-            if (has_byte(hg.sets, u, b)) bucket[b].push_back(u);
+        for (int32_t u : S) for_each_byte(hg.sets, u, [&](int b) { bucket[b].push_back(u); });
         for (int w = 0; w < 8; w++) {
           pairs[2 * w + 1] = (uint32_t)m.keys();
           for (int b = w * 32; b < w * 32 + 32; b++) {
