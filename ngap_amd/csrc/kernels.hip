@@ -257,34 +257,38 @@ NGAP_DEV int32_t memo_rank(const uint32_t *sel, size_t at, uint32_t bit) {
 // independent loads of all levels are issued together (input bytes, then the
 // rank pairs, then the list ranges) and the depth-1 lists are read only for
 // bytes whose list reports.
+// Loads in the root and expansion steps are issued unconditionally, at a
+// valid address when a lane does not need the value (index 0, or the chunk's
+// last byte), and the value is selected afterwards: the loads of a step then
+// go out together instead of one branch (and one wait) at a time.
+NGAP_DEV uint32_t min_u(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
 NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
   const uint8_t *in = P.input;
   const uint32_t p = p0 + W.lane;
   const bool act = p < pend;
   const uint32_t left = act ? W.cend - p : 0u;
   const uint32_t k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
-  const uint32_t b0 = act ? in[p] : 0u;
-  const uint32_t b1 = k >= 2 ? in[p + 1] : 0u;
-  const uint32_t b2 = k >= 3 ? in[p + 2] : 0u;
+  const uint32_t pc = act ? p : p0, last = W.cend - 1;
+  const uint32_t b0 = in[pc];
+  const uint32_t b1 = in[min_u(pc + 1, last)];
+  const uint32_t b2 = in[min_u(pc + 2, last)];
   const bool has = act && ((W.mask1[b0 >> 5] >> (b0 & 31)) & 1u);
   if (wballot(has) == 0) return;
   const uint32_t kReportOnly = 1u << 31;
   if (P.memo_depth < 2) {  // depth 1: all-input states matching b0 become items
-    uint32_t beg = 0, cnt = 0;
-    if (has) {
-      const uint32_t li = W.g * 256u + b0;
-      beg = P.img.beg[0][li];
-      cnt = P.img.beg[0][li + 1] - beg;
-    }
-    flatten(P, W, cnt, beg, p, W.gtag, P.img.list[0]);
+    const uint32_t li = W.g * 256u + b0;
+    const uint32_t beg = P.img.beg[0][li], end = P.img.beg[0][li + 1];
+    flatten(P, W, has ? end - beg : 0u, beg, p, W.gtag, P.img.list[0]);
     return;
   }
   // rank pair of the 2-gram (and the depth-1 reports of rare bytes)
-  int32_t li2 = -1;
-  if (has && k >= 2) {
-    const uint32_t key = (b0 << 8) | b1;
-    li2 = memo_rank(P.img.sel[1], (size_t)W.g * 2048 + (key >> 5), key & 31);
-  }
+  const uint32_t key = (b0 << 8) | b1;
+  const uint2 w2 = reinterpret_cast<const uint2 *>(P.img.sel[1])[(size_t)W.g * 2048 + (key >> 5)];
+  const uint32_t bit2 = key & 31;
+  const int32_t li2 = has && k >= 2 && ((w2.x >> bit2) & 1u)
+                          ? (int32_t)(w2.y + __popc(w2.x & ((1u << bit2) - 1u)))
+                          : -1;
   const bool rep1 = has && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u);
   if (wballot(rep1)) {  // depth 1: reporting entries only (the last position included)
     uint32_t beg = 0, cnt = 0;
@@ -297,19 +301,27 @@ NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) 
   }
   const bool has2 = li2 >= 0;
   if (wballot(has2) == 0) return;
-  // depth-2 list range and the depth-3 rank pair
-  uint32_t beg2 = 0, cnt2 = 0;
+  // depth-2 list range and the depth-3 rank pair (some lane has a depth-2
+  // list, so index 0 and 1 are valid stand-ins)
+  const uint32_t l2 = has2 ? (uint32_t)li2 : 0u;
+  const uint32_t beg2r = P.img.beg[1][l2], end2 = P.img.beg[1][l2 + 1];
+  const uint32_t nrep2 = P.img.nrep[1][l2];
   int32_t li3 = -1;
-  if (has2) {
-    beg2 = P.img.beg[1][li2];
-    cnt2 = k == 2 ? P.img.beg[1][li2 + 1] - beg2 : P.img.nrep[1][li2];
-    if (k != 2) beg2 |= kReportOnly;
-    if (k >= 3) li3 = memo_rank(P.img.sel[2], (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
+  if (P.memo_depth >= 3) {
+    const uint2 w3 = reinterpret_cast<const uint2 *>(P.img.sel[2])[(size_t)l2 * 8 + (b2 >> 5)];
+    const uint32_t bit3 = b2 & 31;
+    li3 = has2 && k >= 3 && ((w3.x >> bit3) & 1u)
+              ? (int32_t)(w3.y + __popc(w3.x & ((1u << bit3) - 1u)))
+              : -1;
   }
+  uint32_t cnt2 = has2 ? (k == 2 ? end2 - beg2r : nrep2) : 0u;
+  const uint32_t beg2 = k == 2 ? beg2r : (beg2r | kReportOnly);
   uint32_t beg3 = 0, cnt3 = 0;
-  if (li3 >= 0) {
-    beg3 = P.img.beg[2][li3];
-    cnt3 = P.img.beg[2][li3 + 1] - beg3;
+  if (wballot(li3 >= 0)) {
+    const uint32_t l3 = li3 >= 0 ? (uint32_t)li3 : 0u;
+    const uint32_t b3 = P.img.beg[2][l3], e3 = P.img.beg[2][l3 + 1];
+    beg3 = b3;
+    cnt3 = li3 >= 0 ? e3 - b3 : 0u;
   }
   flatten(P, W, cnt2, beg2, p + 1, W.gtag, P.img.list[1]);
   if (wballot(cnt3 != 0)) flatten(P, W, cnt3, beg3, p + 2, W.gtag, P.img.list[2]);
@@ -344,11 +356,6 @@ NGAP_DEV void for_entries(const WaveState &W, uint32_t e0, uint32_t ecnt, F &&f)
   }
 }
 
-// Roots of every group for positions [p0, pend) (cross-group work items,
-// memo depth >= 2): one lane per position; the position's byte selects the
-// groups whose all-input states report on it, its 2-gram the depth-2 lists of
-// every group, each continued at depth 3 through that list's rank table.
-// The same lists as roots() visits group by group, so the same items.
 // Per-lane state of one position in roots_cross.
 struct CrossLane {
   uint32_t p, k, b0, e0, ec, t0, tc, r0, rc;
@@ -359,42 +366,49 @@ struct CrossLane {
 NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, uint32_t pend,
                            CrossLane (&L)[2]) {
   const uint8_t *in = P.input;
+  const uint32_t last = W.cend - 1;
   uint32_t b1[2], b2[2];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const uint32_t p = p0 + h * kWave + W.lane;
     const bool act = p < pend;
     const uint32_t left = act ? W.cend - p : 0u;
+    const uint32_t pc = act ? p : p0;
     L[h].p = p;
     L[h].k = left < (uint32_t)P.memo_depth ? left : (uint32_t)P.memo_depth;
-    L[h].b0 = act ? in[p] : 0u;
-    b1[h] = L[h].k >= 2 ? in[p + 1] : 0u;
-    b2[h] = L[h].k >= 3 ? in[p + 2] : 0u;
+    L[h].b0 = in[pc];
+    b1[h] = in[min_u(pc + 1, last)];
+    b2[h] = in[min_u(pc + 2, last)];
+  }
+  uint32_t x2w[2], x2a[2], x2b[2], x3a[2], x3b[2], x1a[2], x1b[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t key2 = (L[h].b0 << 8) | b1[h];
+    x2a[h] = P.img.xr2[key2];
+    x2b[h] = P.img.xr2[key2 + 1];
+    x1a[h] = P.img.xr1[L[h].b0];
+    x1b[h] = P.img.xr1[L[h].b0 + 1];
+    x2w[h] = 0;
+    x3a[h] = x3b[h] = 0;
+    if (P.memo_depth >= 3) {
+      const uint32_t key3 = (key2 << 8) | b2[h];
+      x2w[h] = P.img.x2rep[key2 >> 5];
+      x3a[h] = P.img.xr3[key3];
+      x3b[h] = P.img.xr3[key3 + 1];
+    }
   }
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    const uint32_t k = L[h].k, b0 = L[h].b0;
+    const uint32_t k = L[h].k, b0 = L[h].b0, key2 = (b0 << 8) | b1[h];
     // depth-2 lists: the deepest level at depth 2; at depth 3 only where two
     // bytes are left or the 2-gram has reporting depth-2 lists
-    const uint32_t key2 = (b0 << 8) | b1[h];
-    const bool need2 = k >= 2 && (P.memo_depth == 2 || k == 2 ||
-                                  ((P.img.x2rep[key2 >> 5] >> (key2 & 31)) & 1u));
-    L[h].e0 = L[h].ec = 0;
-    if (need2) {
-      L[h].e0 = P.img.xr2[key2];
-      L[h].ec = P.img.xr2[key2 + 1] - L[h].e0;
-    }
-    L[h].t0 = L[h].tc = 0;
-    if (P.memo_depth >= 3 && k >= 3) {
-      const uint32_t key3 = (key2 << 8) | b2[h];
-      L[h].t0 = P.img.xr3[key3];
-      L[h].tc = P.img.xr3[key3 + 1] - L[h].t0;
-    }
-    L[h].r0 = L[h].rc = 0;
-    if (k >= 1 && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u)) {
-      L[h].r0 = P.img.xr1[b0];
-      L[h].rc = P.img.xr1[b0 + 1] - L[h].r0;
-    }
+    const bool need2 = k >= 2 && (P.memo_depth == 2 || k == 2 || ((x2w[h] >> (key2 & 31)) & 1u));
+    L[h].e0 = x2a[h];
+    L[h].ec = need2 ? x2b[h] - x2a[h] : 0u;
+    L[h].t0 = x3a[h];
+    L[h].tc = P.memo_depth >= 3 && k >= 3 ? x3b[h] - x3a[h] : 0u;
+    L[h].r0 = x1a[h];
+    L[h].rc = k >= 1 && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u) ? x1b[h] - x1a[h] : 0u;
   }
 }
 
@@ -548,12 +562,11 @@ struct LaneGroup {
 NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
                      uint32_t &p, int &depth, const LaneGroup &lg) {
   const uint32_t p1 = p + 1;
-  uint32_t b = 0;
-  Rec rec = empty_rec();
-  if (valid) {
-    b = P.input[p1];
-    rec = load_rec(P, lg.rec_base + v);
-  }
+  // unconditional loads (index 0 stands in for idle lanes; some lane is valid,
+  // so the image has a record)
+  const uint32_t b = P.input[valid ? p1 : 0u];
+  Rec rec = load_rec(P, valid ? lg.rec_base + v : 0u);
+  if (!valid) rec = empty_rec();
   W.items += __popcll(wballot(valid));
   // is p1 the last position of its chunk (no successors enabled after it)?
   // Engine runs have chunks of chunk_len bytes: a divisibility test, valid
@@ -574,10 +587,13 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       if (more && es[s] <= -2) link = -(es[s] + 2);
-      cw[s] = test_word(P, lg.cls_base, more, es[s], ts[s], b);
+      const bool need = more && es[s] >= 0 && !(ts[s] & (kTestLiteral | kTestAll));
+      const size_t at =
+          need ? (size_t)(lg.cls_base + ((uint32_t)es[s] & (uint32_t)kIdMask)) * 8 + (b >> 5) : 0;
+      cw[s] = P.img.cls[at];
     }
-    Rec next = empty_rec();
-    if (more && link >= 0) next = load_rec(P, (uint32_t)link);
+    Rec next = load_rec(P, more && link >= 0 ? (uint32_t)link : 0u);
+    if (!(more && link >= 0)) next = empty_rec();
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int32_t e = es[s];
