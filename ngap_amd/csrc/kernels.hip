@@ -31,6 +31,14 @@
 //    lossy LDS cache (64-bit atomic exchange): an exact duplicate seen
 //    recently is dropped.  A duplicate is never needed (set semantics), and a
 //    first occurrence is never dropped.
+//  * The batched engine is a persistent grid whose waves pull work items
+//    (a window of one chunk, for one group or for all groups) from eight
+//    XCD-local queues; an all-group work item takes the roots of every group
+//    from the cross-group 2-/3-gram index, one lookup per position.
+//  * The loads of a step (input bytes, index offsets, a record and its class
+//    words) are issued unconditionally at valid stand-in addresses and
+//    selected afterwards, so they go out together: the engine is bound by the
+//    latency of dependent gathers.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
