@@ -401,8 +401,9 @@ NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, ui
     if (P.memo_depth >= 3) {
       const uint32_t key3 = (key2 << 8) | b2[h];
       x2w[h] = P.img.x2rep[key2 >> 5];
-      x3a[h] = P.img.xr3[key3];
-      x3b[h] = P.img.xr3[key3 + 1];
+      const uint2 slot = reinterpret_cast<const uint2 *>(P.img.xs3)[key3];
+      x3a[h] = slot.x;
+      x3b[h] = slot.y;
     }
   }
 #pragma unroll
@@ -413,8 +414,9 @@ NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, ui
     const bool need2 = k >= 2 && (P.memo_depth == 2 || k == 2 || ((x2w[h] >> (key2 & 31)) & 1u));
     L[h].e0 = x2a[h];
     L[h].ec = need2 ? x2b[h] - x2a[h] : 0u;
+    // the raw 3-gram slot (decoded in roots_cross)
     L[h].t0 = x3a[h];
-    L[h].tc = P.memo_depth >= 3 && k >= 3 ? x3b[h] - x3a[h] : 0u;
+    L[h].tc = P.memo_depth >= 3 && k >= 3 ? x3b[h] : 0u;
     L[h].r0 = x1a[h];
     L[h].rc = k >= 1 && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u) ? x1b[h] - x1a[h] : 0u;
   }
@@ -477,30 +479,13 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
     }
   }
   if (P.memo_depth < 3) return;
-  // depth 3 from the 3-gram index: (begin in xl3, length | group << 22)
-  const bool any0 = wballot(L[0].tc != 0) != 0, any1 = wballot(L[1].tc != 0) != 0;
-  if (!any0 && !any1) return;
-  if (wballot(L[0].tc > 1 || L[1].tc > 1) == 0) {
-    // the common case: at most one list per lane and half, loaded together
-    uint2 e[2];
+  // depth 3 from the 3-gram slots: an inline state is emitted at once; other
+  // slots point at (begin in xl3, length | group << 22) entries
 #pragma unroll
-    for (int h = 0; h < 2; h++)
-      e[h] = L[h].tc ? reinterpret_cast<const uint2 *>(P.img.xe3)[L[h].t0] : make_uint2(0, 0);
-    if (wballot((e[0].y & (uint32_t)kIdMask) > 1 || (e[1].y & (uint32_t)kIdMask) > 1) == 0) {
-      int32_t x[2];
-#pragma unroll
-      for (int h = 0; h < 2; h++) x[h] = (e[h].y & (uint32_t)kIdMask) ? P.img.xl3[e[h].x] : 0;
-#pragma unroll
-      for (int h = 0; h < 2; h++)
-        emit_one(P, W, (e[h].y & (uint32_t)kIdMask) != 0, x[h], false, L[h].p + 2,
-                 (e[h].y >> 22) << 22);
-      return;
-    }
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-      flatten(P, W, e[h].y & (uint32_t)kIdMask, e[h].x, L[h].p + 2, (e[h].y >> 22) << 22,
-              P.img.xl3);
-    return;
+  for (int h = 0; h < 2; h++) {
+    const bool inl = (L[h].tc & kSlotInline) != 0;
+    emit_one(P, W, inl, (int32_t)L[h].t0, false, L[h].p + 2, (L[h].tc & 1023u) << 22);
+    if (inl) L[h].tc = 0;
   }
 #pragma unroll
   for (int h = 0; h < 2; h++) {

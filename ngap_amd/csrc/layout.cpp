@@ -13,7 +13,7 @@ uint64_t Image::bytes() const {
   for (auto &m : memo)
     b += (m.beg.size() + m.nrep.size() + m.sel.size() + m.list.size()) * 4;
   b += (mask1.size() + xr1.size() + xe1.size() + xmask1.size() + xr2.size() + xe2.size() +
-        xsod.size() + xr3.size() + xe3.size() + xl3.size() + x2rep.size()) * 4;
+        xsod.size() + xs3.size() + xe3.size() + xl3.size() + x2rep.size()) * 4;
   return b;
 }
 
@@ -277,23 +277,38 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     }
   }
   if (memo_depth >= 3) {
-    // 3-gram k -> every group's depth-3 list, lists copied in key order
+    // 3-gram k -> every group's depth-3 list (counting sort by key)
     const MemoLevel &m3 = img.memo[2];
-    img.xr3.assign((1u << 24) + 1, 0);
-    for (uint32_t k : key3) img.xr3[k + 1]++;
-    for (uint32_t k = 0; k < (1u << 24); k++) img.xr3[k + 1] += img.xr3[k];
+    std::vector<uint32_t> start((1u << 24) + 1, 0);
+    for (uint32_t k : key3) start[k + 1]++;
+    for (uint32_t k = 0; k < (1u << 24); k++) start[k + 1] += start[k];
     std::vector<uint32_t> order(key3.size());
     {
-      std::vector<uint32_t> at(img.xr3.begin(), img.xr3.end() - 1);
+      std::vector<uint32_t> at(start.begin(), start.end() - 1);
       for (size_t i = 0; i < key3.size(); i++) order[at[key3[i]]++] = (uint32_t)i;
     }
-    img.xe3.resize(key3.size() * 2);
+    img.xs3.assign(2u << 24, 0);
+    img.xe3.reserve(key3.size() * 2);
     img.xl3.reserve(m3.list.size());
-    for (size_t j = 0; j < order.size(); j++) {
-      const uint32_t li = order[j];
-      img.xe3[2 * j] = (uint32_t)img.xl3.size();
-      img.xe3[2 * j + 1] = (m3.beg[li + 1] - m3.beg[li]) | (grp3[li] << 22);
-      img.xl3.insert(img.xl3.end(), m3.list.begin() + m3.beg[li], m3.list.begin() + m3.beg[li + 1]);
+    for (uint32_t k = 0; k < (1u << 24); k++) {
+      const uint32_t n = start[k + 1] - start[k];
+      if (n == 0) continue;
+      const uint32_t li0 = order[start[k]];
+      if (n == 1 && m3.beg[li0 + 1] - m3.beg[li0] == 1) {
+        // one group, one state: the slot holds the state itself
+        img.xs3[2 * (size_t)k] = (uint32_t)m3.list[m3.beg[li0]];
+        img.xs3[2 * (size_t)k + 1] = kSlotInline | grp3[li0];
+        continue;
+      }
+      img.xs3[2 * (size_t)k] = (uint32_t)(img.xe3.size() / 2);
+      img.xs3[2 * (size_t)k + 1] = n;
+      for (uint32_t j = start[k]; j < start[k + 1]; j++) {
+        const uint32_t li = order[j];
+        img.xe3.push_back((uint32_t)img.xl3.size());
+        img.xe3.push_back((m3.beg[li + 1] - m3.beg[li]) | (grp3[li] << 22));
+        img.xl3.insert(img.xl3.end(), m3.list.begin() + m3.beg[li],
+                       m3.list.begin() + m3.beg[li + 1]);
+      }
     }
     // 2-grams with a reporting depth-2 list (their reports come from depth 2)
     img.x2rep.assign(2048, 0);

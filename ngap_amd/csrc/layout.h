@@ -46,6 +46,7 @@ constexpr int kMaxMemoDepth = 3;
 constexpr int kRecWords = 8;                  // int32 words per successor record
 constexpr uint32_t kTestLiteral = 1u << 31;   // successor test: one byte (bits 0-7)
 constexpr uint32_t kTestAll = 1u << 30;       // successor test: every byte
+constexpr uint32_t kSlotInline = 1u << 31;    // 3-gram slot holds its one state
 
 struct HostGroup {  // what the C ABI hands in for one group
   int32_t num_states = 0;
@@ -105,12 +106,15 @@ struct Image {
   //       xr2[k+1]-1 of xe2, each 4 words (list index, list begin, list end,
   //       reporting entries | group << 22), one 16-byte load per entry;
   //   xsod: (tagged id, group) pairs of every group's start-of-data states.
-  //   x3 (depth 3): 3-gram k -> every group's depth-3 list: entries xr3[k]
-  //       .. xr3[k+1]-1 of xe3, each (begin in xl3, length | group << 22),
-  //       the lists copied into xl3 in key order (one position reads one
-  //       offset pair and then contiguous entries and lists); x2rep marks the
-  //       2-grams that have a reporting depth-2 list (2048 words).
-  std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod, xr3, xe3, x2rep;
+  //   x3 (depth 3): 3-gram k -> every group's depth-3 list, through the
+  //       8-byte slot xs3[2k], xs3[2k+1]: (0, 0) when no group has one;
+  //       (tagged state, kSlotInline | group) when exactly one group has one
+  //       and it holds one state (the common case: one load, no list);
+  //       otherwise (first entry, entry count) into xe3, whose entries are
+  //       (begin in xl3, length | group << 22) with the lists copied into xl3
+  //       in key order.  x2rep marks the 2-grams that have a reporting
+  //       depth-2 list (2048 words).
+  std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod, xs3, xe3, x2rep;
   std::vector<int32_t> xl3;
   // statistics (for the learned scheduler's features and reports)
   std::vector<uint32_t> stat_edges, stat_aas, stat_sod, stat_report,

@@ -141,7 +141,7 @@ KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_op
   P.img.xr2 = eng->d_xr2.as<uint32_t>();
   P.img.xe2 = eng->d_xe2.as<uint32_t>();
   P.img.xsod = eng->d_xsod.as<uint32_t>();
-  P.img.xr3 = eng->d_xr3.as<uint32_t>();
+  P.img.xs3 = eng->d_xs3.as<uint32_t>();
   P.img.xe3 = eng->d_xe3.as<uint32_t>();
   P.img.xl3 = eng->d_xl3.as<int32_t>();
   P.img.x2rep = eng->d_x2rep.as<uint32_t>();
@@ -604,7 +604,7 @@ int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t nu
   HIP_TRY(eng->d_xr2.upload(im.xr2));
   HIP_TRY(eng->d_xe2.upload(im.xe2));
   HIP_TRY(eng->d_xsod.upload(im.xsod));
-  HIP_TRY(eng->d_xr3.upload(im.xr3));
+  HIP_TRY(eng->d_xs3.upload(im.xs3));
   HIP_TRY(eng->d_xe3.upload(im.xe3));
   HIP_TRY(eng->d_xl3.upload(im.xl3));
   HIP_TRY(eng->d_x2rep.upload(im.x2rep));

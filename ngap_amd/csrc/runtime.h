@@ -88,7 +88,7 @@ struct ngap_engine {
   ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;
   ngapamd::DevBuf d_beg[ngapamd::kMaxMemoDepth], d_nrep[ngapamd::kMaxMemoDepth],
       d_list[ngapamd::kMaxMemoDepth], d_sel[ngapamd::kMaxMemoDepth];
-  ngapamd::DevBuf d_xr1, d_xe1, d_xmask1, d_xr2, d_xe2, d_xsod, d_xr3, d_xe3, d_xl3, d_x2rep;
+  ngapamd::DevBuf d_xr1, d_xe1, d_xmask1, d_xr2, d_xe2, d_xsod, d_xs3, d_xe3, d_xl3, d_x2rep;
   ngapamd::DevBuf d_input;
   uint64_t chunk_len = 0;
   uint32_t num_seg = 0;

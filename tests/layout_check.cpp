@@ -181,16 +181,26 @@ int main() {
             const int li3 = rank_lookup(im.memo[2].sel, (size_t)li2 * 8 + (b2 >> 5), b2 & 31);
             if (want3.empty() != (li3 < 0)) fail("depth 3 presence");
             else if (li3 >= 0 && list_of(im.memo[2], li3, nullptr) != want3) fail("depth 3 list");
-            // the 3-gram cross-group index: this group's entry, its list
+            // the 3-gram cross-group slot: this group's list, inline or
+            // through its entry
             const uint32_t k3 = (key << 8) | (uint32_t)b2;
+            const uint32_t s0 = im.xs3[2 * (size_t)k3], s1 = im.xs3[2 * (size_t)k3 + 1];
             int found = 0;
-            for (uint32_t i = im.xr3[k3]; i < im.xr3[k3 + 1]; i++) {
-              if ((im.xe3[2 * i + 1] >> 22) != (uint32_t)gi) continue;
-              found++;
-              std::set<int> got;
-              const uint32_t beg = im.xe3[2 * i], cnt = im.xe3[2 * i + 1] & kIdMask;
-              for (uint32_t j = beg; j < beg + cnt; j++) got.insert(im.xl3[j] & kIdMask);
-              if (got != want3) fail("xl3 list");
+            if (s1 & kSlotInline) {
+              if ((s1 & 1023u) == (uint32_t)gi) {
+                found++;
+                if (want3 != std::set<int>{(int)(s0 & kIdMask)}) fail("inline slot");
+              }
+            } else {
+              for (uint32_t i = s0; i < s0 + s1; i++) {
+                if ((im.xe3[2 * i + 1] >> 22) != (uint32_t)gi) continue;
+                found++;
+                std::set<int> got;
+                const uint32_t beg = im.xe3[2 * i], cnt = im.xe3[2 * i + 1] & kIdMask;
+                for (uint32_t j = beg; j < beg + cnt; j++) got.insert(im.xl3[j] & kIdMask);
+                if (got != want3) fail("xl3 list");
+                if (s1 == 1 && cnt == 1) fail("a one-state slot not inline");
+              }
             }
             if (found != (want3.empty() ? 0 : 1)) fail("xe3 entry");
           }
