@@ -1134,14 +1134,9 @@ hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint
   return hipGetLastError();
 }
 
-hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds) {
-  // the grid is sized for the kernel with the lower occupancy of the two
-  int a = 0, b = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batched, kBlockThreads, lds);
-  if (e != hipSuccess) return e;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batched_stream, kBlockThreads, lds);
-  *blocks_per_cu = a < b ? a : b;
-  return e;
+hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, bool stream) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, stream ? k_batched_stream : k_batched, kBlockThreads, lds);
 }
 
 hipError_t set_lds_limit(uint32_t lds) {

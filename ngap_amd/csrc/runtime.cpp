@@ -28,7 +28,7 @@ hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long lon
 hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
 hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
-hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds);
+hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, bool stream);
 hipError_t set_lds_limit(uint32_t lds);
 
 hipError_t DevBuf::reserve(size_t bytes) {
@@ -171,11 +171,11 @@ void attach_ctx(KParams &P, RunCtx &ctx) {
 }
 
 int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks,
-                      uint32_t ring) {
-  const int slot = 31 - __builtin_clz(ring);  // log2, ring in [64, 2048]
+                      uint32_t ring, bool stream) {
+  const int slot = 2 * (31 - __builtin_clz(ring)) + (stream ? 1 : 0);  // log2, ring in [64, 2048]
   int per_cu = eng->batched_per_cu[slot];
   if (per_cu == 0) {
-    HIP_TRY(batched_occupancy(&per_cu, lds_bytes(ring)));
+    HIP_TRY(batched_occupancy(&per_cu, lds_bytes(ring), stream));
     const_cast<ngap_engine *>(eng)->batched_per_cu[slot] = per_cu;
   }
   if (per_cu < 1) per_cu = 1;
@@ -244,7 +244,9 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   ctx.last_traces = o.task_traces != 0;
   P.overflow = ctx.overflow.as<uint32_t>();
   uint32_t blocks = 0;
-  int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap);
+  // engine runs without traces launch the streaming kernel (launch_batched)
+  int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap,
+                             P.chunk_len && !P.task_items);
   if (rc) return rc;
   P.spill_cap = o.spill_capacity ? o.spill_capacity : 65536;
   HIP_TRY(ctx.spill.reserve((size_t)blocks * P.waves_per_block * P.spill_cap * 8));
@@ -682,7 +684,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   uint32_t waves = 0;
   if (kind != 0) {
     uint32_t blocks = 0;
-    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap);
+    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap, !o.task_traces);
     if (rc) return rc;
     waves = blocks * P.waves_per_block;
   }
@@ -753,7 +755,7 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
     if (axis == 2) {
       ngap_run_opts f = cur;
       f.persistent_blocks = 0;
-      int rc2 = persistent_blocks(eng, 0, &full, ring_of(f));
+      int rc2 = persistent_blocks(eng, 0, &full, ring_of(f), !f.task_traces);
       if (rc2) return rc2;
     }
     const std::vector<uint32_t> grids = {full / 4, full / 2, (full * 3) / 4};
@@ -798,7 +800,7 @@ int ngap_batched_grid(ngap_engine *eng, uint32_t ring_capacity, uint32_t *blocks
   const uint32_t ring = ring_of(o);
   if (!ring) return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
   HIP_TRY(hipSetDevice(eng->device));
-  int rc = persistent_blocks(eng, 0, blocks, ring);
+  int rc = persistent_blocks(eng, 0, blocks, ring, true);
   if (rc) return rc;
   *waves_per_block = kBlockThreads / kWave;
   return NGAP_OK;

@@ -82,7 +82,7 @@ uint32_t lds_bytes(uint32_t ring = kRing);
 struct ngap_engine {
   int device = 0;
   int num_cus = 0;
-  int batched_per_cu[12] = {0};  // k_batched blocks per CU by log2(ring), 0 = not queried
+  int batched_per_cu[24] = {0};  // blocks per CU by (log2(ring), streaming kernel), 0 = not queried
   ngapamd::Image img;
   bool loaded = false;
   ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;
@@ -113,7 +113,7 @@ namespace ngapamd {
 KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_opts &o);
 void attach_ctx(KParams &P, RunCtx &ctx);
 int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks,
-                      uint32_t ring = kRing);
+                      uint32_t ring = kRing, bool stream = true);
 uint32_t ring_of(const ngap_run_opts &o);
 // Runs tasks on ctx (batched kernel), with the overflow fallback of o.
 int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> &tasks,

@@ -289,7 +289,7 @@ int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
   uint32_t blocks = 0;
   if (!ring_of(*opts))
     return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
-  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts));
+  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts), false);
   if (rc) return rc;
   // every launch gets the whole persistent grid: launches in flight share the
   // CUs, and a lone launch is not held to a fraction of the GPU
