@@ -343,7 +343,7 @@ def main():
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
                         traffic_gbs=round(traffic / (kern * 1e-3) / 1e9, 1) if traffic else None,
-                        kernel="k_batched",
+                        kernel="k_batched_stream",
                         work_items="all-group" if cross else "per-group",
                         algorithmic_bytes_per_input_byte=round(model, 3),
                         per_group_model_bytes_per_input_byte=round(per_byte, 3),
