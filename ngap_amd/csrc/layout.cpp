@@ -2,8 +2,10 @@
 #include "layout.h"
 
 #include <algorithm>
+#include <atomic>
 #include <bitset>
 #include <sstream>
+#include <thread>
 
 namespace ngapamd {
 
@@ -46,6 +48,171 @@ void push_list(MemoLevel &m, std::vector<int32_t> &ids, const std::vector<int32_
   m.beg.push_back((uint32_t)m.list.size());
 }
 
+// One group's share of the image, with indices local to the group (record
+// links, list offsets, key ranks); build_image rebases them when it
+// concatenates the groups.  Groups are independent, so parts build in
+// parallel.
+struct GroupPart {
+  bool ok = true;
+  std::string err;
+  GroupMeta gm{};
+  std::vector<int32_t> succ4;        // records; links -(local index + 2)
+  std::vector<int32_t> sod;          // tagged start-of-data ids
+  uint32_t mask1[16] = {0};
+  MemoLevel memo[kMaxMemoDepth];     // beg from 0, sel ranks local
+  std::vector<uint32_t> key2, key3;  // keys of the depth-2 / depth-3 lists
+  uint32_t edges = 0, wide = 0, selfloop = 0, report = 0;
+};
+
+bool fail_part(GroupPart &P, const std::string &m) {
+  P.ok = false;
+  P.err = m;
+  return false;
+}
+
+bool build_part(const HostGroup &hg, int32_t g, int memo_depth, GroupPart &P) {
+  const int n = hg.num_states;
+  if (n < 0 || n > kIdMask + 1) {
+    std::ostringstream o;
+    o << "group " << g << " has " << n << " states; state ids are limited to 22 bits";
+    return fail_part(P, o.str());
+  }
+  if (n > 0 && (!hg.row || !hg.sets || !hg.attrs)) return fail_part(P, "group arrays missing");
+  const int E = n > 0 ? hg.row[n] : 0;
+  for (int v = 0; v < n; v++)
+    if (hg.row[v] > hg.row[v + 1] || hg.row[v] < 0)
+      return fail_part(P, "CSR row offsets are not monotone");
+  for (int e = 0; e < E; e++)
+    if (hg.col[e] < 0 || hg.col[e] >= n) return fail_part(P, "CSR column index out of range");
+  std::vector<char> is_aas(n, 0);
+  for (int i = 0; i < hg.num_aas; i++) {
+    if (hg.aas[i] < 0 || hg.aas[i] >= n) return fail_part(P, "all-input id out of range");
+    is_aas[hg.aas[i]] = 1;
+  }
+  for (int i = 0; i < hg.num_sod; i++)
+    if (hg.sod[i] < 0 || hg.sod[i] >= n) return fail_part(P, "start-of-data id out of range");
+
+  // successor lists without all-input successors (distinct), in-degrees
+  std::vector<std::vector<int32_t>> succ(n);
+  std::vector<int32_t> indeg(n, 0);
+  for (int v = 0; v < n; v++) {
+    for (int e = hg.row[v]; e < hg.row[v + 1]; e++) {
+      int u = hg.col[e];
+      if (u == v) P.selfloop++;
+      if (is_aas[u]) continue;
+      if (std::find(succ[v].begin(), succ[v].end(), u) == succ[v].end()) succ[v].push_back(u);
+    }
+    std::sort(succ[v].begin(), succ[v].end());
+    for (int u : succ[v]) indeg[u]++;
+    int cnt = 0;
+    for (int w = 0; w < 8; w++) cnt += __builtin_popcount(hg.sets[(size_t)v * 8 + w]);
+    if (cnt >= 128) P.wide++;
+    if (hg.attrs[v] & 2) P.report++;
+  }
+  P.edges = (uint32_t)E;
+  std::vector<int32_t> tag(n);
+  for (int v = 0; v < n; v++) {
+    int32_t t = v;
+    if (hg.attrs[v] & 2) t |= kFlagReport;
+    if (succ[v].empty()) t |= kFlagDeadEnd;
+    if (indeg[v] >= 2) t |= kFlagDedup;
+    tag[v] = t;
+  }
+
+  // successor test words: a one-byte class is tested by comparison, an
+  // all-byte class always matches, any other class reads its class word
+  std::vector<int32_t> test(n, 0);
+  for (int v = 0; v < n; v++) {
+    int cnt = 0, last = 0;
+    for (int w = 0; w < 8; w++) {
+      const uint32_t x = hg.sets[(size_t)v * 8 + w];
+      if (x) last = w * 32 + 31 - __builtin_clz(x);
+      cnt += __builtin_popcount(x);
+    }
+    if (cnt == 1) test[v] = (int32_t)(kTestLiteral | (uint32_t)last);
+    else if (cnt == 256) test[v] = (int32_t)kTestAll;
+  }
+
+  P.gm.num_states = n;
+  P.gm.num_aas = hg.num_aas;
+  // records: n primary records, continuation records appended
+  auto new_records = [&](size_t count) {
+    const size_t at = P.succ4.size();
+    P.succ4.resize(at + count * kRecWords, -1);
+    for (size_t r = 0; r < count; r++)
+      for (int i = 4; i < 8; i++) P.succ4[at + r * kRecWords + i] = 0;
+    return at;
+  };
+  new_records((size_t)n);
+  for (int v = 0; v < n; v++) {
+    const auto &s = succ[v];
+    size_t rec = (size_t)v * kRecWords;
+    size_t k = 0;
+    for (;;) {
+      size_t left = s.size() - k;
+      const size_t here = left <= 4 ? left : 3;
+      for (size_t i = 0; i < here; i++) {
+        P.succ4[rec + i] = tag[s[k + i]];
+        P.succ4[rec + 4 + i] = test[s[k + i]];
+      }
+      if (left <= 4) break;
+      k += 3;
+      const size_t next = new_records(1);
+      P.succ4[rec + 3] = -(int32_t)(next / kRecWords) - 2;
+      rec = next;
+    }
+  }
+  for (int i = 0; i < hg.num_sod; i++) P.sod.push_back(tag[hg.sod[i]]);
+
+  for (int d = 0; d < kMaxMemoDepth; d++) P.memo[d].beg.push_back(0);
+  // memo depth 1: all-input states by byte (all 256 keys present)
+  std::vector<std::vector<int32_t>> l1(256);
+  for (int i = 0; i < hg.num_aas; i++)
+    for_each_byte(hg.sets, hg.aas[i], [&](int b) { l1[b].push_back(hg.aas[i]); });
+  for (int b = 0; b < 256; b++) {
+    std::sort(l1[b].begin(), l1[b].end());
+    l1[b].erase(std::unique(l1[b].begin(), l1[b].end()), l1[b].end());
+    if (!l1[b].empty()) P.mask1[b / 32] |= 1u << (b % 32);
+    push_list(P.memo[0], l1[b], tag);
+    if (P.memo[0].nrep.back()) P.mask1[8 + b / 32] |= 1u << (b % 32);
+  }
+  // deeper levels, each list's successors split by the next byte
+  std::vector<char> mark(n, 0);
+  std::vector<std::vector<int32_t>> bucket(256);
+  std::vector<std::vector<int32_t>> prev_lists(l1.begin(), l1.end());
+  for (int d = 2; d <= memo_depth; d++) {
+    MemoLevel &m = P.memo[d - 1];
+    // depth 2: 2048 pairs (256 first bytes x 8); depth 3: 8 pairs per depth-2 list
+    m.sel.assign(prev_lists.size() * 16, 0u);
+    std::vector<std::vector<int32_t>> cur_lists;
+    for (size_t li = 0; li < prev_lists.size(); li++) {
+      // key bits: depth 2 uses (first byte li, second byte b) inside the
+      // group's 65536-key block; depth 3 uses the 256 keys below list li
+      uint32_t *pairs = &m.sel[li * 16];
+      std::vector<int32_t> S;
+      for (int32_t v : prev_lists[li])
+        for (int32_t u : succ[v])
+          if (!mark[u]) { mark[u] = 1; S.push_back(u); }
+      for (int32_t u : S) mark[u] = 0;
+      for (auto &bk : bucket) bk.clear();
+      for (int32_t u : S) for_each_byte(hg.sets, u, [&](int b) { bucket[b].push_back(u); });
+      for (int w = 0; w < 8; w++) {
+        pairs[2 * w + 1] = (uint32_t)m.keys();  // local rank, rebased on merge
+        for (int b = w * 32; b < w * 32 + 32; b++) {
+          if (bucket[b].empty()) continue;
+          pairs[2 * w] |= 1u << (b & 31);
+          if (d == 2) P.key2.push_back(((uint32_t)li << 8) | (uint32_t)b);
+          else P.key3.push_back((P.key2[li] << 8) | (uint32_t)b);
+          cur_lists.push_back(bucket[b]);
+          push_list(m, bucket[b], tag);
+        }
+      }
+    }
+    prev_lists.swap(cur_lists);
+  }
+  return true;
+}
+
 }  // namespace
 
 bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth,
@@ -66,183 +233,74 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
   std::vector<uint32_t> key2, grp2;  // 2-gram and group of every depth-2 list
   std::vector<uint32_t> key3, grp3;  // 3-gram and group of every depth-3 list
 
+  // the groups' parts, built in parallel
+  std::vector<GroupPart> parts(num_groups);
+  {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nthreads = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)num_groups);
+    std::atomic<int32_t> next(0);
+    auto worker = [&]() {
+      for (int32_t g; (g = next.fetch_add(1)) < num_groups;) build_part(groups[g], g, memo_depth, parts[g]);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+  }
+  for (int32_t g = 0; g < num_groups; g++)
+    if (!parts[g].ok) {
+      err = parts[g].err;
+      return false;
+    }
+  // concatenation, rebasing the part-local indices
   for (int32_t g = 0; g < num_groups; g++) {
+    GroupPart &P = parts[g];
     const HostGroup &hg = groups[g];
     const int n = hg.num_states;
-    if (n < 0 || n > kIdMask + 1) {
-      std::ostringstream o;
-      o << "group " << g << " has " << n << " states; state ids are limited to 22 bits";
-      err = o.str();
-      return false;
-    }
-    if (n > 0 && (!hg.row || !hg.sets || !hg.attrs)) {
-      err = "group arrays missing";
-      return false;
-    }
-    const int E = n > 0 ? hg.row[n] : 0;
-    for (int v = 0; v < n; v++)
-      if (hg.row[v] > hg.row[v + 1] || hg.row[v] < 0) {
-        err = "CSR row offsets are not monotone";
-        return false;
-      }
-    for (int e = 0; e < E; e++)
-      if (hg.col[e] < 0 || hg.col[e] >= n) {
-        err = "CSR column index out of range";
-        return false;
-      }
-    std::vector<char> is_aas(n, 0);
-    for (int i = 0; i < hg.num_aas; i++) {
-      if (hg.aas[i] < 0 || hg.aas[i] >= n) { err = "all-input id out of range"; return false; }
-      is_aas[hg.aas[i]] = 1;
-    }
-    for (int i = 0; i < hg.num_sod; i++)
-      if (hg.sod[i] < 0 || hg.sod[i] >= n) { err = "start-of-data id out of range"; return false; }
-
-    // successor lists without all-input successors (distinct), in-degrees
-    std::vector<std::vector<int32_t>> succ(n);
-    std::vector<int32_t> indeg(n, 0);
-    uint32_t wide = 0, selfloop = 0, report = 0;
-    for (int v = 0; v < n; v++) {
-      for (int e = hg.row[v]; e < hg.row[v + 1]; e++) {
-        int u = hg.col[e];
-        if (u == v) selfloop++;
-        if (is_aas[u]) continue;
-        if (std::find(succ[v].begin(), succ[v].end(), u) == succ[v].end()) succ[v].push_back(u);
-      }
-      std::sort(succ[v].begin(), succ[v].end());
-      for (int u : succ[v]) indeg[u]++;
-      int cnt = 0;
-      for (int w = 0; w < 8; w++) cnt += __builtin_popcount(hg.sets[(size_t)v * 8 + w]);
-      if (cnt >= 128) wide++;
-      if (hg.attrs[v] & 2) report++;
-    }
-    std::vector<int32_t> tag(n);
-    for (int v = 0; v < n; v++) {
-      int32_t t = v;
-      if (hg.attrs[v] & 2) t |= kFlagReport;
-      if (succ[v].empty()) t |= kFlagDeadEnd;
-      if (indeg[v] >= 2) t |= kFlagDedup;
-      tag[v] = t;
-    }
-
-    // successor test words: a one-byte class is tested by comparison, an
-    // all-byte class always matches, any other class reads its class word
-    std::vector<int32_t> test(n, 0);
-    for (int v = 0; v < n; v++) {
-      int cnt = 0, last = 0;
-      for (int w = 0; w < 8; w++) {
-        const uint32_t x = hg.sets[(size_t)v * 8 + w];
-        if (x) last = w * 32 + 31 - __builtin_clz(x);
-        cnt += __builtin_popcount(x);
-      }
-      if (cnt == 1) test[v] = (int32_t)(kTestLiteral | (uint32_t)last);
-      else if (cnt == 256) test[v] = (int32_t)kTestAll;
-    }
-
-    GroupMeta gm{};
-    gm.rec_base = (uint32_t)(img.succ4.size() / kRecWords);
+    GroupMeta gm = P.gm;
+    const uint32_t rec_base = (uint32_t)(img.succ4.size() / kRecWords);
+    gm.rec_base = rec_base;
     gm.cls_base = (uint32_t)(img.cls.size() / 8);
-    gm.num_states = n;
-    gm.num_aas = hg.num_aas;
-    // records: n primary records, continuation records appended
-    auto new_records = [&](size_t count) {
-      const size_t at = img.succ4.size();
-      img.succ4.resize(at + count * kRecWords, -1);
-      for (size_t r = 0; r < count; r++)
-        for (int i = 4; i < 8; i++) img.succ4[at + r * kRecWords + i] = 0;
-      return at;
-    };
-    const size_t prim = new_records((size_t)n);
-    for (int v = 0; v < n; v++) {
-      const auto &s = succ[v];
-      size_t rec = prim + (size_t)v * kRecWords;
-      size_t k = 0;
-      for (;;) {
-        size_t left = s.size() - k;
-        const size_t here = left <= 4 ? left : 3;
-        for (size_t i = 0; i < here; i++) {
-          img.succ4[rec + i] = tag[s[k + i]];
-          img.succ4[rec + 4 + i] = test[s[k + i]];
-        }
-        if (left <= 4) break;
-        k += 3;
-        const size_t next = new_records(1);
-        img.succ4[rec + 3] = -(int32_t)(next / kRecWords) - 2;
-        rec = next;
-      }
-    }
+    for (size_t r = 0; r < P.succ4.size(); r += kRecWords)
+      if (P.succ4[r + 3] <= -2) P.succ4[r + 3] = -(int32_t)(rec_base - P.succ4[r + 3] - 2) - 2;
+    img.succ4.insert(img.succ4.end(), P.succ4.begin(), P.succ4.end());
     img.cls.insert(img.cls.end(), hg.sets, hg.sets + (size_t)n * 8);
     gm.sod_begin = (uint32_t)img.sod.size();
-    for (int i = 0; i < hg.num_sod; i++) img.sod.push_back(tag[hg.sod[i]]);
+    img.sod.insert(img.sod.end(), P.sod.begin(), P.sod.end());
     gm.sod_end = (uint32_t)img.sod.size();
-    for (int i = 0; i < hg.num_sod; i++) {
-      img.xsod.push_back((uint32_t)tag[hg.sod[i]]);
+    for (int32_t t : P.sod) {
+      img.xsod.push_back((uint32_t)t);
       img.xsod.push_back((uint32_t)g);
     }
-
-    // memo depth 1: all-input states by byte (all 256 keys present)
-    std::vector<std::vector<int32_t>> l1(256);
-    for (int i = 0; i < hg.num_aas; i++)
-      for_each_byte(hg.sets, hg.aas[i], [&](int b) { l1[b].push_back(hg.aas[i]); });
-    size_t m1 = img.mask1.size();
-    img.mask1.resize(m1 + 16, 0u);
-    for (int b = 0; b < 256; b++) {
-      std::sort(l1[b].begin(), l1[b].end());
-      l1[b].erase(std::unique(l1[b].begin(), l1[b].end()), l1[b].end());
-      if (!l1[b].empty()) img.mask1[m1 + b / 32] |= 1u << (b % 32);
-      push_list(img.memo[0], l1[b], tag);
-      if (img.memo[0].nrep.back()) img.mask1[m1 + 8 + b / 32] |= 1u << (b % 32);
+    img.mask1.insert(img.mask1.end(), P.mask1, P.mask1 + 16);
+    for (int d = 0; d < kMaxMemoDepth; d++) {
+      MemoLevel &m = img.memo[d];
+      const MemoLevel &pm = P.memo[d];
+      const uint32_t list_base = (uint32_t)m.list.size(), key_base = (uint32_t)m.keys();
+      for (size_t i = 1; i < pm.beg.size(); i++) m.beg.push_back(list_base + pm.beg[i]);
+      m.nrep.insert(m.nrep.end(), pm.nrep.begin(), pm.nrep.end());
+      m.list.insert(m.list.end(), pm.list.begin(), pm.list.end());
+      const size_t s0 = m.sel.size();
+      m.sel.insert(m.sel.end(), pm.sel.begin(), pm.sel.end());
+      for (size_t i = s0 + 1; i < m.sel.size(); i += 2) m.sel[i] += key_base;
     }
-    // deeper levels, each list's successors split by the next byte
-    std::vector<char> mark(n, 0);
-    std::vector<std::vector<int32_t>> bucket(256);
-    std::vector<std::vector<int32_t>> prev_lists;  // lists of previous depth
-    for (int b = 0; b < 256; b++) prev_lists.push_back(l1[b]);
-    const size_t key2_base = key2.size();  // this group's depth-2 lists in key2
-    for (int d = 2; d <= memo_depth; d++) {
-      MemoLevel &m = img.memo[d - 1];
-      // depth 2: 2048 pairs per group; depth 3: 8 pairs per depth-2 key
-      const size_t sel0 = m.sel.size();
-      m.sel.resize(sel0 + prev_lists.size() * 16, 0u);
-      std::vector<std::vector<int32_t>> cur_lists;
-      for (size_t li = 0; li < prev_lists.size(); li++) {
-        // key bits: depth 2 uses (first byte li, second byte b) inside the
-        // group's 65536-key block; depth 3 uses the 256 keys below list li
-        uint32_t *pairs = &m.sel[sel0 + li * 16];
-        std::vector<int32_t> S;
-        for (int32_t v : prev_lists[li])
-          for (int32_t u : succ[v])
-            if (!mark[u]) { mark[u] = 1; S.push_back(u); }
-        for (int32_t u : S) mark[u] = 0;
-        for (auto &bk : bucket) bk.clear();
-        for (int32_t u : S) for_each_byte(hg.sets, u, [&](int b) { bucket[b].push_back(u); });
-        for (int w = 0; w < 8; w++) {
-          pairs[2 * w + 1] = (uint32_t)m.keys();
-          for (int b = w * 32; b < w * 32 + 32; b++) {
-            if (bucket[b].empty()) continue;
-            pairs[2 * w] |= 1u << (b & 31);
-            if (d == 2) {
-              key2.push_back(((uint32_t)li << 8) | (uint32_t)b);
-              grp2.push_back((uint32_t)g);
-            } else if (d == 3) {
-              key3.push_back((key2[key2_base + li] << 8) | (uint32_t)b);
-              grp3.push_back((uint32_t)g);
-            }
-            cur_lists.push_back(bucket[b]);
-            push_list(m, bucket[b], tag);
-          }
-        }
-      }
-      prev_lists.swap(cur_lists);
+    for (uint32_t k : P.key2) {
+      key2.push_back(k);
+      grp2.push_back((uint32_t)g);
+    }
+    for (uint32_t k : P.key3) {
+      key3.push_back(k);
+      grp3.push_back((uint32_t)g);
     }
     img.meta.push_back(gm);
-    img.stat_edges.push_back((uint32_t)E);
+    img.stat_edges.push_back(P.edges);
     img.stat_aas.push_back(hg.num_aas);
     img.stat_sod.push_back(hg.num_sod);
-    img.stat_report.push_back(report);
-    img.stat_wide.push_back(wide);
-    img.stat_selfloop.push_back(selfloop);
+    img.stat_report.push_back(P.report);
+    img.stat_wide.push_back(P.wide);
+    img.stat_selfloop.push_back(P.selfloop);
     img.total_states += n;
+    P = GroupPart();  // release as we go
   }
   // cross-group index: counting sort of the lists by byte / 2-gram
   img.xr1.assign(257, 0);

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
 def test_image_tables_match_brute_force(tmp_path):
     exe = tmp_path / "layout_check"
-    subprocess.run(["g++", "-std=c++17", "-O2", "-o", str(exe),
+    subprocess.run(["g++", "-std=c++17", "-O2", "-pthread", "-o", str(exe),
                     os.path.join(ROOT, "tests", "layout_check.cpp"),
                     os.path.join(ROOT, "ngap_amd", "csrc", "layout.cpp")], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
