@@ -111,6 +111,14 @@ typedef struct {
   int32_t task_traces;          /* batched: record the items of every work item
                                    (ngap_get_task_items); a wave then finishes
                                    one work item before it starts the next */
+  int32_t root_pass;            /* batched engine runs: 1 = compute every work
+                                   item's roots in a pass of their own (many
+                                   short waves) before the persistent kernel
+                                   expands them, 0 = roots inside the
+                                   persistent kernel */
+  uint32_t root_capacity;       /* root pass: root items reserved per input
+                                   position of a work item, 0 = 4; a work item
+                                   with more is re-run by the fallback */
 } ngap_run_opts;
 
 typedef struct {
@@ -142,11 +150,11 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats)
 
 /* Autotuner of the batched engine (the reference picks these knobs by hand
  * per application, code/scripts/configs/exec_config_ngap_groups_best): times
- * candidate work-item windows, LDS worklist capacities, persistent grid sizes
- * and worklist spill capacities on the loaded input by coordinate descent (at
- * most max_runs timed runs) and writes the fastest into *best: a copy of
- * *base, batched, with fallback on, window / ring_capacity /
- * persistent_blocks / spill_capacity set.  Report
+ * candidate work-item windows, LDS worklist capacities, persistent grid sizes,
+ * worklist spill capacities and the root pass (on or off) on the loaded input
+ * by coordinate descent (at most max_runs timed runs) and writes the fastest
+ * into *best: a copy of *base, batched, with fallback on, window /
+ * ring_capacity / persistent_blocks / spill_capacity / root_pass set.  Report
  * sets do not depend on these knobs; the chunk length is part of the input's
  * meaning (matches do not cross chunks) and is not tuned. */
 int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,

@@ -71,6 +71,8 @@ class RunOpts(C.Structure):
         ("cross_groups", C.c_int32),
         ("ring_capacity", C.c_uint32),
         ("task_traces", C.c_int32),
+        ("root_pass", C.c_int32),
+        ("root_capacity", C.c_uint32),
     ]
 
 

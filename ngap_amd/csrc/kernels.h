@@ -36,9 +36,20 @@ struct Task {
   uint32_t group, cstart, cend, begin, end, flags;
 };
 constexpr uint32_t kTaskSeedStart = 1;
+// Work-item overflow flags: the worklist spill area filled up (the item's
+// results are incomplete: fallback or error), or its root-pass area did (it
+// is re-run without a root pass).
+constexpr uint32_t kSpillOverflow = 1, kRootOverflow = 2;
 // Task.group of a work item that scans its window for every group at once
 // (roots from the cross-group index, memo depth >= 2).
 constexpr uint32_t kAllGroups = 0xFFFFFFFFu;
+
+// A piece of a work item for the root pass (k_roots): roots at positions
+// [begin, end) of work item `task`; flags as Task.flags (the first piece of a
+// work item carries its start-of-data seeding).
+struct RootPiece {
+  uint32_t task, begin, end, flags;
+};
 
 struct KParams {
   DevImage img;
@@ -68,6 +79,15 @@ struct KParams {
   uint32_t *task_head;     // kQueues claim counters, kQueueStride words apart
   unsigned long long *task_items;  // nullable, per task
   uint32_t waves_per_block;
+  // Root pass (nullable root_items = off): k_roots writes the root items of
+  // work item t to root_items[root_off[t] .. root_off[t + 1]), counting them
+  // in root_count[t]; the streaming kernel then expands them instead of
+  // computing roots.
+  const RootPiece *pieces;
+  uint32_t num_pieces;
+  const uint64_t *root_off;
+  uint64_t *root_items;
+  unsigned long long *root_count;
 };
 
 constexpr uint32_t kWave = 64;

@@ -82,6 +82,10 @@ struct WaveState {
   uint32_t lane;
   bool overflow;
   unsigned long long items;
+  // root pass: the work item's area of root_items and its item count
+  uint64_t *out;
+  unsigned long long *out_count;
+  uint64_t out_cap;
   // current (group, chunk)
   uint32_t g, gtag, rec_base, cls_base, cstart, cend;
 };
@@ -131,11 +135,36 @@ NGAP_DEV bool test_hit(bool live, int32_t e, uint32_t t, uint32_t word, uint32_t
   return (word >> (b & 31)) & 1u;
 }
 
+// Root pass: the wave's ring is a staging buffer (filled from 0), appended
+// to the work item's area of root_items when it fills and at the end of a
+// piece.  Items beyond the area's capacity are dropped and the work item is
+// marked for the fallback.
+NGAP_DEV void flush_stage(const KParams &P, WaveState &W) {
+  if (W.tail == 0) return;
+  wsync();
+  unsigned long long base = 0;
+  if (W.lane == 0) base = atomicAdd(W.out_count, (unsigned long long)W.tail);
+  base = __shfl(base, 0, 64);
+  for (uint32_t i = W.lane; i < W.tail; i += kWave)
+    if (base + i < W.out_cap) W.out[base + i] = W.ring[i];
+  if (base + W.tail > W.out_cap) W.overflow = true;
+  W.tail = 0;
+  wsync();
+}
+
+template <bool Stage = false>
 NGAP_DEV void push(const KParams &P, WaveState &W, bool pred, uint64_t item) {
   uint64_t m = wballot(pred);
   if (m == 0) return;
   uint32_t cnt = __popcll(m);
   uint32_t r = mask_rank(m);
+  if (Stage) {
+    if (W.tail + cnt > P.ring_cap) flush_stage(P, W);
+    if (pred) W.ring[W.tail + r] = item;
+    W.tail += cnt;
+    wsync();
+    return;
+  }
   if (W.tail - W.head + cnt <= P.ring_cap) {
     if (pred) W.ring[(W.tail + r) & (P.ring_cap - 1)] = item;
     W.tail += cnt;
@@ -199,6 +228,7 @@ NGAP_DEV void refill_from_spill(const KParams &P, WaveState &W) {
 // reports is emitted; entries are also pushed as items unless the lane's
 // list is report-only (bit 31 of beg), the state has no successors, or the
 // position is the chunk's last.
+template <bool Stage = false>
 NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg,
                       uint32_t pos, uint32_t gtag, const int32_t *list) {
   if (wballot(cnt > 1u) == 0) {
@@ -210,7 +240,7 @@ NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg
     emit(P, W, ok && (e & kFlagReport), it);
     bool keep = ok && !(beg >> 31) && !(e & kFlagDeadEnd) && (pos + 1 < W.cend);
     if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
-    push(P, W, keep, it);
+    push<Stage>(P, W, keep, it);
     return;
   }
   uint32_t incl = wave_incl_scan(cnt, W.lane);
@@ -245,7 +275,7 @@ NGAP_DEV void flatten(const KParams &P, WaveState &W, uint32_t cnt, uint32_t beg
     emit(P, W, ok && (e & kFlagReport), item_of(u, ppos));
     bool keep = ok && !ronly && !(e & kFlagDeadEnd) && (ppos + 1 < W.cend);
     if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, item_of(u, ppos));
-    push(P, W, keep, item_of(u, ppos));
+    push<Stage>(P, W, keep, item_of(u, ppos));
   }
   wsync();
 }
@@ -271,6 +301,7 @@ NGAP_DEV int32_t memo_rank(const uint32_t *sel, size_t at, uint32_t bit) {
 // go out together instead of one branch (and one wait) at a time.
 NGAP_DEV uint32_t min_u(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+template <bool Stage = false>
 NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
   const uint8_t *in = P.input;
   const uint32_t p = p0 + W.lane;
@@ -287,7 +318,7 @@ NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) 
   if (P.memo_depth < 2) {  // depth 1: all-input states matching b0 become items
     const uint32_t li = W.g * 256u + b0;
     const uint32_t beg = P.img.beg[0][li], end = P.img.beg[0][li + 1];
-    flatten(P, W, has ? end - beg : 0u, beg, p, W.gtag, P.img.list[0]);
+    flatten<Stage>(P, W, has ? end - beg : 0u, beg, p, W.gtag, P.img.list[0]);
     return;
   }
   // rank pair of the 2-gram (and the depth-1 reports of rare bytes)
@@ -305,7 +336,7 @@ NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) 
       beg = P.img.beg[0][li] | kReportOnly;
       cnt = P.img.nrep[0][li];
     }
-    flatten(P, W, cnt, beg, p, W.gtag, P.img.list[0]);
+    flatten<Stage>(P, W, cnt, beg, p, W.gtag, P.img.list[0]);
   }
   const bool has2 = li2 >= 0;
   if (wballot(has2) == 0) return;
@@ -331,8 +362,8 @@ NGAP_DEV void roots(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) 
     beg3 = b3;
     cnt3 = li3 >= 0 ? e3 - b3 : 0u;
   }
-  flatten(P, W, cnt2, beg2, p + 1, W.gtag, P.img.list[1]);
-  if (wballot(cnt3 != 0)) flatten(P, W, cnt3, beg3, p + 2, W.gtag, P.img.list[2]);
+  flatten<Stage>(P, W, cnt2, beg2, p + 1, W.gtag, P.img.list[1]);
+  if (wballot(cnt3 != 0)) flatten<Stage>(P, W, cnt3, beg3, p + 2, W.gtag, P.img.list[2]);
 }
 
 // Distributes per-lane index ranges [e0, e0 + ecnt) over the wave, 64
@@ -424,13 +455,14 @@ NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, ui
 
 // One entry per lane at most: the entry and its single list item are loaded
 // for both halves before either is emitted.
+template <bool Stage = false>
 NGAP_DEV void emit_one(const KParams &P, WaveState &W, bool ok, int32_t e, bool ronly,
                        uint32_t pos, uint32_t tag) {
   const uint64_t it = item_of(((uint32_t)e & (uint32_t)kIdMask) | tag, pos);
   emit(P, W, ok && (e & kFlagReport), it);
   bool keep = ok && !ronly && !(e & kFlagDeadEnd) && (pos + 1 < W.cend);
   if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
-  push(P, W, keep, it);
+  push<Stage>(P, W, keep, it);
 }
 
 // Roots of every group for positions [p0, pend), pend - p0 <= 128 (cross-
@@ -438,6 +470,7 @@ NGAP_DEV void emit_one(const KParams &P, WaveState &W, bool ok, int32_t e, bool 
 // position's byte selects the groups whose all-input states report on it,
 // its 2-gram the depth-2 lists and its 3-gram the depth-3 lists of every
 // group.  The same lists as roots() visits group by group, so the same items.
+template <bool Stage = false>
 NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t pend) {
   const uint32_t kReportOnly = 1u << 31;
   CrossLane L[2];
@@ -456,7 +489,7 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
           cnt = P.img.nrep[0][li];
           tag = (li >> 8) << 22;
         }
-        flatten(P, W, cnt, beg, pos, tag, P.img.list[0]);
+        flatten<Stage>(P, W, cnt, beg, pos, tag, P.img.list[0]);
       });
     }
     // depth 2: full lists where it is the deepest level, reports otherwise
@@ -474,7 +507,7 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
           cnt = ko == 2 || P.memo_depth == 2 ? e.z - e.y : (e.w & (uint32_t)kIdMask);
           if (!(ko == 2 || P.memo_depth == 2)) beg |= kReportOnly;
         }
-        flatten(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
+        flatten<Stage>(P, W, cnt, beg, pos + 1, tag, P.img.list[1]);
       });
     }
   }
@@ -484,7 +517,7 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const bool inl = (L[h].tc & kSlotInline) != 0;
-    emit_one(P, W, inl, (int32_t)L[h].t0, false, L[h].p + 2, (L[h].tc & 1023u) << 22);
+    emit_one<Stage>(P, W, inl, (int32_t)L[h].t0, false, L[h].p + 2, (L[h].tc & 1023u) << 22);
     if (inl) L[h].tc = 0;
   }
 #pragma unroll
@@ -500,11 +533,12 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
         cnt = e.y & (uint32_t)kIdMask;
         tag = (e.y >> 22) << 22;
       }
-      flatten(P, W, cnt, beg, pos + 2, tag, P.img.xl3);
+      flatten<Stage>(P, W, cnt, beg, pos + 2, tag, P.img.xl3);
     });
   }
 }
 
+template <bool Stage = false>
 NGAP_DEV void seed_start_of_data(const KParams &P, WaveState &W, uint32_t sod_begin,
                                  uint32_t sod_end) {
   uint32_t b = P.input[W.cstart];
@@ -518,12 +552,13 @@ NGAP_DEV void seed_start_of_data(const KParams &P, WaveState &W, uint32_t sod_be
     emit(P, W, hit && (e & kFlagReport), it);
     bool keep = hit && !(e & kFlagDeadEnd) && (W.cstart + 1 < W.cend);
     if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
-    push(P, W, keep, it);
+    push<Stage>(P, W, keep, it);
   }
 }
 
 // The same for every group (cross-group work items): (tagged id, group)
 // pairs, each group's class base from its metadata.
+template <bool Stage = false>
 NGAP_DEV void seed_start_all(const KParams &P, WaveState &W) {
   const uint32_t b = P.input[W.cstart];
   for (uint32_t i = 0; i < P.img.n_xsod; i += kWave) {
@@ -541,7 +576,7 @@ NGAP_DEV void seed_start_all(const KParams &P, WaveState &W) {
     emit(P, W, hit && (e & kFlagReport), it);
     bool keep = hit && !(e & kFlagDeadEnd) && (W.cstart + 1 < W.cend);
     if (keep && P.dedup && (e & kFlagDedup)) keep = dedup_fresh(P, W, it);
-    push(P, W, keep, it);
+    push<Stage>(P, W, keep, it);
   }
 }
 
@@ -671,6 +706,9 @@ NGAP_DEV void setup_wave(const KParams &P, WaveState &W, unsigned char *base,
   W.open = W.mask1 + 16;
   W.n_open = 0;
   W.spill = spill;
+  W.out = nullptr;
+  W.out_count = nullptr;
+  W.out_cap = 0;
   W.head = W.tail = W.rn = W.spill_n = 0;
   W.lane = lane_id();
   W.overflow = false;
@@ -751,10 +789,12 @@ NGAP_DEV uint32_t claim_task(const KParams &P, const WaveState &W, uint32_t &q) 
 // work items to finish, 2 = the queue is empty.
 struct TaskStream {
   uint32_t cur = 0, end = 0;
+  uint64_t rbase = 0;       // root pass: the work item's first root item
   bool cross = false;
   bool cross_mask = false;  // LDS holds the cross-group report mask
   uint32_t q = 0;           // the wave's queue
   NGAP_DEV void start(const KParams &P, const WaveState &W) { q = blockIdx.x % kQueues; }
+  template <bool RootItems>
   NGAP_DEV int next(const KParams &P, WaveState &W, uint32_t &a, uint32_t &b) {
     if (cur >= end) {
       if (W.n_open >= kMaxOpen) return 1;
@@ -764,6 +804,17 @@ struct TaskStream {
       const Task T = P.tasks[ti];
       if (W.lane == 0) W.open[W.n_open] = ti;
       W.n_open++;
+      if (RootItems) {  // the root pass wrote this work item's roots
+        rbase = P.root_off[ti];
+        const uint64_t cap = P.root_off[ti + 1] - rbase;
+        const unsigned long long n = P.root_count[ti];
+        cur = 0;
+        end = (uint32_t)(n < cap ? n : cap);
+        a = cur;
+        b = end < kWave ? end : kWave;
+        cur = b;
+        return 0;
+      }
       cross = T.group == kAllGroups;
       if (cross) {
         if (!cross_mask) set_cross(P, W, T.cstart, T.cend);
@@ -785,7 +836,8 @@ struct TaskStream {
       end = T.end;
     }
     a = cur;
-    const uint32_t step = cross ? 2 * kWave : kWave;  // roots_cross takes two halves
+    // roots_cross takes two halves; the root pass's items go 64 at a time
+    const uint32_t step = cross && !RootItems ? 2 * kWave : kWave;
     b = cur + step < end ? cur + step : end;
     cur = b;
     return 0;
@@ -799,7 +851,7 @@ NGAP_DEV void close_open(const KParams &P, WaveState &W) {
   if (W.n_open == 0) return;
   wsync();
   if (W.overflow) {
-    if (W.lane < W.n_open) P.overflow[W.open[W.lane]] = 1u;
+    if (W.lane < W.n_open) atomicOr(&P.overflow[W.open[W.lane]], kSpillOverflow);
     if (W.lane == 0) atomicAdd(P.overflow_count, W.n_open);
   }
   W.n_open = 0;
@@ -807,8 +859,17 @@ NGAP_DEV void close_open(const KParams &P, WaveState &W) {
   wsync();
 }
 
+// Queues root items [from, from + n) of the root pass (n <= 64).
+NGAP_DEV void take_roots(const KParams &P, WaveState &W, uint64_t from, uint32_t n) {
+  if (n == 0) return;
+  const bool ok = W.lane < n;
+  const uint64_t it = P.root_items[ok ? from + W.lane : from];
+  push(P, W, ok, it);
+}
+
 // drain() over a TaskStream: items of several work items (and groups) share
 // the queue, so a lane looks up its item's group bases when it takes one.
+template <bool RootItems>
 NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
   TaskStream src;
   src.start(P, W);
@@ -825,9 +886,10 @@ NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
     }
     if (qn < kWave && W.spill_n == 0 && more) {
       uint32_t a, b;
-      const int st = src.next(P, W, a, b);
+      const int st = src.template next<RootItems>(P, W, a, b);
       if (st == 0) {
-        if (src.cross) roots_cross(P, W, a, b);
+        if (RootItems) take_roots(P, W, src.rbase + a, b - a);
+        else if (src.cross) roots_cross(P, W, a, b);
         else roots(P, W, a, b);
         continue;
       }
@@ -968,9 +1030,63 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched_stream(KParams P) {
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
   clear_dedup(P, W);
-  drain_stream(P, W);
+  drain_stream<false>(P, W);
   flush_reports(P, W);
   if (W.lane == 0) atomicAdd(P.items, W.items);
+}
+
+// The streaming kernel after a root pass (k_roots): its work items' roots
+// are read from root_items, so it holds only the expansion's registers.
+__global__ void __launch_bounds__(kBlockThreads) k_expand(KParams P) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t wbytes = wave_lds_bytes(P.ring_cap, P.rbuf_cap, P.dc_cap);
+  WaveState W;
+  setup_wave(P, W, lds + 16 + wave * wbytes,
+             P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
+  clear_dedup(P, W);
+  drain_stream<true>(P, W);
+  flush_reports(P, W);
+  if (W.lane == 0) atomicAdd(P.items, W.items);
+}
+
+// Root pass of an engine run: one wave per piece of a work item (a non-
+// persistent grid, so the gathers of many waves overlap).  The piece's roots
+// (and start-of-data seeds) are staged in the wave's LDS ring and appended
+// to the work item's area of root_items; their reports go out at once.
+// k_batched_stream then expands the items.
+__global__ void __launch_bounds__(kBlockThreads) k_roots(KParams P) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t piece = blockIdx.x * P.waves_per_block + wave;
+  if (piece >= P.num_pieces) return;  // wave-uniform; no block barrier follows
+  const uint32_t wbytes = wave_lds_bytes(P.ring_cap, P.rbuf_cap, P.dc_cap);
+  WaveState W;
+  setup_wave(P, W, lds + 16 + wave * wbytes, nullptr);
+  const RootPiece R = P.pieces[piece];
+  const Task T = P.tasks[R.task];
+  const uint64_t o0 = P.root_off[R.task];
+  W.out = P.root_items + o0;
+  W.out_cap = P.root_off[R.task + 1] - o0;
+  W.out_count = P.root_count + R.task;
+  if (T.group == kAllGroups) {
+    begin_cross(P, W, T.cstart, T.cend);
+    if (R.flags & kTaskSeedStart) seed_start_all<true>(P, W);
+    for (uint32_t a = R.begin; a < R.end; a += 2 * kWave)
+      roots_cross<true>(P, W, a, min_u(a + 2 * kWave, R.end));
+  } else {
+    begin_unit(P, W, T.group, T.cstart, T.cend);
+    if (R.flags & kTaskSeedStart) {
+      const GroupMeta gm = P.img.meta[T.group];
+      seed_start_of_data<true>(P, W, gm.sod_begin, gm.sod_end);
+    }
+    for (uint32_t a = R.begin; a < R.end; a += kWave)
+      roots<true>(P, W, a, min_u(a + kWave, R.end));
+  }
+  flush_stage(P, W);
+  flush_reports(P, W);
+  if (W.lane == 0 && W.overflow && atomicOr(&P.overflow[R.task], kRootOverflow) == 0u)
+    atomicAdd(P.overflow_count, 1u);
 }
 
 // blockinggroups (the reference's BAP baseline, kernel_ngap_O0.cu:7-126): one
@@ -1093,7 +1209,8 @@ __global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
 // re-run of overflowed work items keeps the report count (keep_reports).
 __global__ void k_reset(unsigned long long *report_count, unsigned long long *items,
                         uint32_t *task_head, uint32_t *overflow, uint32_t n_overflow,
-                        uint32_t *overflow_count, int keep_reports) {
+                        uint32_t *overflow_count, int keep_reports,
+                        unsigned long long *root_count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     if (!keep_reports) *report_count = 0;
@@ -1101,18 +1218,21 @@ __global__ void k_reset(unsigned long long *report_count, unsigned long long *it
     for (uint32_t k = 0; k < kQueues; k++) task_head[k * kQueueStride] = 0;
     *overflow_count = 0;
   }
-  for (; i < n_overflow; i += gridDim.x * blockDim.x) overflow[i] = 0;
+  for (; i < n_overflow; i += gridDim.x * blockDim.x) {
+    overflow[i] = 0;
+    if (root_count) root_count[i] = 0;
+  }
 }
 
 // Host-side launch wrappers (this translation unit owns the kernels).
 hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
                         uint32_t *head, uint32_t *ovf, uint32_t n, uint32_t *ovf_count,
-                        int keep_reports) {
+                        int keep_reports, unsigned long long *root_count) {
   uint32_t blocks = (n + 255) / 256;
   if (blocks == 0) blocks = 1;
   if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_reset, dim3(blocks), dim3(256), 0, s, rc, items, head, ovf, n,
-                     ovf_count, keep_reports);
+                     ovf_count, keep_reports, root_count);
   return hipGetLastError();
 }
 
@@ -1126,17 +1246,27 @@ hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint
   return hipGetLastError();
 }
 
+hipError_t launch_roots(hipStream_t s, const KParams &P, uint32_t lds) {
+  const uint32_t blocks = (P.num_pieces + P.waves_per_block - 1) / P.waves_per_block;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_roots, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+  return hipGetLastError();
+}
+
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds) {
-  if (P.chunk_len && !P.task_items)
+  if (P.chunk_len && !P.task_items && P.root_items)
+    hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+  else if (P.chunk_len && !P.task_items)
     hipLaunchKernelGGL(k_batched_stream, dim3(blocks), dim3(kBlockThreads), lds, s, P);
   else
     hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);
   return hipGetLastError();
 }
 
-hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, bool stream) {
+hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, int kind) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, stream ? k_batched_stream : k_batched, kBlockThreads, lds);
+      blocks_per_cu, kind == 2 ? k_expand : kind == 1 ? k_batched_stream : k_batched,
+      kBlockThreads, lds);
 }
 
 hipError_t set_lds_limit(uint32_t lds) {
@@ -1147,6 +1277,12 @@ hipError_t set_lds_limit(uint32_t lds) {
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched_stream),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_roots),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_expand),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),

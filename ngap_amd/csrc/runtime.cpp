@@ -22,15 +22,6 @@ int ngap_fail(int code, const std::string &msg) {
 
 namespace ngapamd {
 
-hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
-                        uint32_t *head, uint32_t *ovf, uint32_t n, uint32_t *ovf_count,
-                        int keep_reports);
-hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
-hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
-hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
-hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, bool stream);
-hipError_t set_lds_limit(uint32_t lds);
-
 hipError_t DevBuf::reserve(size_t bytes) {
   if (bytes <= n && p) return hipSuccess;
   release();
@@ -170,16 +161,63 @@ void attach_ctx(KParams &P, RunCtx &ctx) {
   P.report_cap = ctx.report_cap;
 }
 
+// Kernel kind an engine run of o launches: 0 k_batched (traces), 1
+// k_batched_stream, 2 k_expand (after the root pass).
+int engine_kind(const ngap_run_opts &o) { return o.task_traces ? 0 : o.root_pass ? 2 : 1; }
+
 int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks,
-                      uint32_t ring, bool stream) {
-  const int slot = 2 * (31 - __builtin_clz(ring)) + (stream ? 1 : 0);  // log2, ring in [64, 2048]
+                      uint32_t ring, int kind) {
+  const int slot = 3 * (31 - __builtin_clz(ring)) + kind;  // log2, ring in [64, 2048]
   int per_cu = eng->batched_per_cu[slot];
   if (per_cu == 0) {
-    HIP_TRY(batched_occupancy(&per_cu, lds_bytes(ring), stream));
+    HIP_TRY(batched_occupancy(&per_cu, lds_bytes(ring), kind));
     const_cast<ngap_engine *>(eng)->batched_per_cu[slot] = per_cu;
   }
   if (per_cu < 1) per_cu = 1;
   *blocks = requested ? requested : (uint32_t)(per_cu * eng->num_cus);
+  return NGAP_OK;
+}
+
+// Root pass set-up for the resident work items: each work item's area of
+// root items (factor items per position, the start-of-data seeds of a seeding
+// work item and 64 spare) and its pieces of at most kRootPiece positions.
+// Returns false when the areas would exceed kRootBudget bytes (no root pass).
+constexpr uint32_t kRootPiece = 512;
+constexpr uint64_t kRootBudget = 16ull << 30;
+
+static int root_areas(const ngap_engine *eng, RunCtx &ctx, const std::vector<Task> &tasks,
+                      uint32_t factor, bool same_tasks, bool *ok) {
+  *ok = false;
+  if (same_tasks && ctx.root_factor == factor && ctx.root_off.p) {
+    *ok = true;
+    return NGAP_OK;
+  }
+  ctx.root_factor = 0;
+  const uint64_t n_xsod = eng->img.xsod.size() / 2;
+  std::vector<uint64_t> off(tasks.size() + 1, 0);
+  std::vector<RootPiece> pieces;
+  for (size_t t = 0; t < tasks.size(); t++) {
+    const Task &T = tasks[t];
+    const bool seed = (T.flags & kTaskSeedStart) != 0;
+    uint64_t sods = 0;
+    if (seed)
+      sods = T.group == kAllGroups ? n_xsod
+                                   : eng->img.meta[T.group].sod_end - eng->img.meta[T.group].sod_begin;
+    off[t + 1] = off[t] + (uint64_t)(T.end - T.begin) * factor + sods + kWave;
+    if (T.begin == T.end && seed) pieces.push_back({(uint32_t)t, T.begin, T.end, T.flags});
+    for (uint32_t a = T.begin; a < T.end; a += kRootPiece) {
+      const uint32_t b = T.end - a > kRootPiece ? a + kRootPiece : T.end;
+      pieces.push_back({(uint32_t)t, a, b, a == T.begin ? T.flags : 0u});
+    }
+  }
+  if (off.back() * 8 > kRootBudget || pieces.size() >= (1u << 31)) return NGAP_OK;
+  HIP_TRY(ctx.root_off.upload(off));
+  HIP_TRY(ctx.pieces.upload(pieces));
+  HIP_TRY(ctx.root_items.reserve(std::max<uint64_t>(off.back(), 1) * 8));
+  HIP_TRY(ctx.root_count.reserve(std::max<size_t>(tasks.size(), 1) * 8));
+  ctx.num_pieces = (uint32_t)pieces.size();
+  ctx.root_factor = factor;
+  *ok = true;
   return NGAP_OK;
 }
 
@@ -198,11 +236,13 @@ static int launch_batched_ctx(RunCtx &ctx, const KParams &P, uint32_t blocks, ui
       ctx.drop_graph();
       HIP_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
       hipError_t e1 = launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
-                                   n_flags, P.overflow_count, keep_reports);
+                                   n_flags, P.overflow_count, keep_reports, P.root_count);
+      hipError_t e4 = P.root_items ? launch_roots(ctx.stream, P, lds) : hipSuccess;
       hipError_t e2 = launch_batched(ctx.stream, P, blocks, lds);
       hipGraph_t g = nullptr;
       hipError_t e3 = hipStreamEndCapture(ctx.stream, &g);
       HIP_TRY(e1);
+      HIP_TRY(e4);
       HIP_TRY(e2);
       HIP_TRY(e3);
       ctx.graph = g;
@@ -213,7 +253,8 @@ static int launch_batched_ctx(RunCtx &ctx, const KParams &P, uint32_t blocks, ui
     return NGAP_OK;
   }
   HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, n_flags,
-                       P.overflow_count, keep_reports));
+                       P.overflow_count, keep_reports, P.root_count));
+  if (P.root_items) HIP_TRY(launch_roots(ctx.stream, P, lds));
   HIP_TRY(launch_batched(ctx.stream, P, blocks, lds));
   return NGAP_OK;
 }
@@ -243,10 +284,30 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   P.task_items = o.task_traces ? ctx.task_items.as<unsigned long long>() : nullptr;
   ctx.last_traces = o.task_traces != 0;
   P.overflow = ctx.overflow.as<uint32_t>();
+  // root pass: engine runs on the streaming kernel only
+  int rc = NGAP_OK;
+  if (o.root_pass && P.chunk_len && !P.task_items && T) {
+    bool ok = false;
+    if (ctx.root_gen != eng->generation) {
+      ctx.root_gen = eng->generation;
+      ctx.root_boost = 1;
+    }
+    const uint32_t factor = (o.root_capacity ? o.root_capacity : 4) * ctx.root_boost;
+    rc = root_areas(eng, ctx, tasks, factor, same, &ok);
+    if (rc) return rc;
+    if (ok) {
+      P.pieces = ctx.pieces.as<RootPiece>();
+      P.num_pieces = ctx.num_pieces;
+      P.root_off = ctx.root_off.as<uint64_t>();
+      P.root_items = ctx.root_items.as<uint64_t>();
+      P.root_count = ctx.root_count.as<unsigned long long>();
+    }
+  }
   uint32_t blocks = 0;
-  // engine runs without traces launch the streaming kernel (launch_batched)
-  int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap,
-                             P.chunk_len && !P.task_items);
+  // engine runs without traces launch the streaming kernel (k_expand after a
+  // root pass), the others the per-work-item kernel (launch_batched)
+  rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap,
+                         P.root_items ? 2 : (P.chunk_len && !P.task_items ? 1 : 0));
   if (rc) return rc;
   P.spill_cap = o.spill_capacity ? o.spill_capacity : 65536;
   HIP_TRY(ctx.spill.reserve((size_t)blocks * P.waves_per_block * P.spill_cap * 8));
@@ -263,11 +324,18 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   unsigned long long cnt[4] = {0, 0, 0, 0};
   HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 32, hipMemcpyDeviceToHost));
   std::vector<uint32_t> bad;
+  bool spill_over = false;  // some work item lost items (not only its root area)
   if (cnt[3] && T) {  // per-item flags only when some work item overflowed
     std::vector<uint32_t> flags(T);
     HIP_TRY(hipMemcpy(flags.data(), P.overflow, T * 4ull, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < T; i++)
+    bool root_over = false;
+    for (uint32_t i = 0; i < T; i++) {
       if (flags[i]) bad.push_back(i);
+      spill_over |= (flags[i] & kSpillOverflow) != 0;
+      root_over |= (flags[i] & kRootOverflow) != 0;
+    }
+    // later runs reserve larger root areas
+    if (root_over) ctx.root_boost = std::min<uint32_t>(ctx.root_boost * 2, 64);
   }
   if (st) {
     st->kernel_ms = ms;
@@ -277,15 +345,18 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   }
   // Overflow fallback: re-run only the overflowed work items, on a small grid
   // with a worklist 16x larger each round; their earlier partial reports stay
-  // (the set union is exact).
+  // (the set union is exact).  Work items whose root-pass area filled up are
+  // re-run (without a root pass) even when the fallback is off, at the same
+  // worklist capacity then.
   uint32_t spill = P.spill_cap;
   double fb_ms = 0;
   uint32_t fb_tasks = 0;
-  for (int round = 0; !bad.empty() && o.fallback && round < 3; round++) {
+  for (int round = 0; !bad.empty() && round < 3 && (o.fallback || (round == 0 && !spill_over));
+       round++) {
     std::vector<Task> redo;
     for (uint32_t i : bad) redo.push_back(tasks[i]);
     fb_tasks += (uint32_t)redo.size();
-    spill = (uint32_t)std::min<uint64_t>((uint64_t)spill * 16, 1u << 24);
+    if (o.fallback) spill = (uint32_t)std::min<uint64_t>((uint64_t)spill * 16, 1u << 24);
     uint32_t fb_blocks = std::min<uint32_t>(blocks, ((uint32_t)redo.size() + 3) / 4);
     const size_t per_block = (size_t)P.waves_per_block * spill * 8;
     if ((size_t)fb_blocks * per_block > (32ull << 30))
@@ -298,6 +369,12 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
     F.task_items = nullptr;
     F.spill = ctx.fb_spill.as<uint64_t>();
     F.spill_cap = spill;
+    // the re-run computes its roots in the persistent kernel
+    F.pieces = nullptr;
+    F.num_pieces = 0;
+    F.root_off = nullptr;
+    F.root_items = nullptr;
+    F.root_count = nullptr;
     HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
     rc = launch_batched_ctx(ctx, F, fb_blocks, F.num_tasks, true, false);
     if (rc) return rc;
@@ -491,7 +568,7 @@ int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
   attach_ctx(P, ctx);
   HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
   HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, units,
-                       P.overflow_count, 0));
+                       P.overflow_count, 0, nullptr));
   if (blocking) HIP_TRY(launch_blocking(ctx.stream, P, units, lds_bytes()));
   else HIP_TRY(launch_plain(ctx.stream, P, units, lds_bytes()));
   HIP_TRY(hipEventRecord(ctx.ev1, ctx.stream));
@@ -684,7 +761,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   uint32_t waves = 0;
   if (kind != 0) {
     uint32_t blocks = 0;
-    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap, !o.task_traces);
+    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap, engine_kind(o));
     if (rc) return rc;
     waves = blocks * P.waves_per_block;
   }
@@ -749,18 +826,19 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
   const std::vector<uint32_t> windows = {256, 512, 1024, 2048, 4096, 8192, 16384};
   const std::vector<uint32_t> rings = {128, 512, 1024};
   const std::vector<uint32_t> spills = {8192, 16384, 262144};
-  for (int axis = 0; axis < 4; axis++) {
+  for (int axis = 0; axis < 5; axis++) {
     // the grid axis scales the full grid of the ring capacity picked so far
     uint32_t full = 0;
     if (axis == 2) {
       ngap_run_opts f = cur;
       f.persistent_blocks = 0;
-      int rc2 = persistent_blocks(eng, 0, &full, ring_of(f), !f.task_traces);
+      int rc2 = persistent_blocks(eng, 0, &full, ring_of(f), engine_kind(f));
       if (rc2) return rc2;
     }
     const std::vector<uint32_t> grids = {full / 4, full / 2, (full * 3) / 4};
-    const std::vector<uint32_t> &vals =
-        axis == 0 ? windows : axis == 1 ? rings : axis == 2 ? grids : spills;
+    const std::vector<uint32_t> passes = {cur.root_pass ? 0u : 1u};
+    const std::vector<uint32_t> &vals = axis == 0 ? windows : axis == 1 ? rings : axis == 2 ? grids
+                                        : axis == 3 ? spills : passes;
     ngap_run_opts pick = cur;
     for (uint32_t v : vals) {
       if (runs >= max_runs) break;
@@ -773,8 +851,10 @@ int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
       } else if (axis == 2) {
         if (v == 0) continue;
         c.persistent_blocks = v;
-      } else {
+      } else if (axis == 3) {
         c.spill_capacity = v;
+      } else {
+        c.root_pass = (int32_t)v;
       }
       double t = 0;
       rc = timeit(c, t);
@@ -800,7 +880,7 @@ int ngap_batched_grid(ngap_engine *eng, uint32_t ring_capacity, uint32_t *blocks
   const uint32_t ring = ring_of(o);
   if (!ring) return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
   HIP_TRY(hipSetDevice(eng->device));
-  int rc = persistent_blocks(eng, 0, blocks, ring, true);
+  int rc = persistent_blocks(eng, 0, blocks, ring, engine_kind(o));
   if (rc) return rc;
   *waves_per_block = kBlockThreads / kWave;
   return NGAP_OK;

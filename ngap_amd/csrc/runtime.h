@@ -53,6 +53,13 @@ struct RunCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf reports, counters, overflow, spill, tasks, task_items, fb_tasks, fb_spill;
+  // root pass: pieces of the resident work items, each work item's area
+  // offsets (items), the areas and the per-work-item counts
+  DevBuf pieces, root_off, root_items, root_count;
+  uint32_t num_pieces = 0;
+  uint32_t root_factor = 0;     // items per position the resident areas were sized for
+  uint32_t root_boost = 1;      // doubles (up to 64x) after a root area overflowed,
+  uint64_t root_gen = 0;        // for this engine generation (groups and input)
   uint64_t report_cap = 0, last_count = 0;
   bool last_report_off = false, have_run = false;
   bool tasks_resident = false;  // `tasks` holds last_tasks
@@ -77,12 +84,25 @@ struct Mapping {
 bool map_algorithm(int algorithm, int depth_req, int loaded_depth, Mapping &m, std::string &err);
 uint32_t lds_bytes(uint32_t ring = kRing);
 
+// Kernel launchers (kernels.hip).  launch_reset clears the run's counters
+// and n per-work-item flags (and root counts, when given); launch_roots runs
+// the root pass (k_roots) over P.pieces.
+hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
+                        uint32_t *head, uint32_t *ovf, uint32_t n, uint32_t *ovf_count,
+                        int keep_reports, unsigned long long *root_count);
+hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
+hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
+hipError_t launch_roots(hipStream_t s, const KParams &P, uint32_t lds);
+hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
+hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, int kind);  // 0 k_batched, 1 k_batched_stream, 2 k_expand
+hipError_t set_lds_limit(uint32_t lds);
+
 }  // namespace ngapamd
 
 struct ngap_engine {
   int device = 0;
   int num_cus = 0;
-  int batched_per_cu[24] = {0};  // blocks per CU by (log2(ring), streaming kernel), 0 = not queried
+  int batched_per_cu[36] = {0};  // blocks per CU by (log2(ring), kernel kind), 0 = not queried
   ngapamd::Image img;
   bool loaded = false;
   ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;
@@ -112,8 +132,10 @@ struct ngap_engine {
 namespace ngapamd {
 KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_opts &o);
 void attach_ctx(KParams &P, RunCtx &ctx);
+int engine_kind(const ngap_run_opts &o);
+// Persistent grid of the batched kernel `kind` (batched_occupancy) for ring.
 int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks,
-                      uint32_t ring = kRing, bool stream = true);
+                      uint32_t ring = kRing, int kind = 1);
 uint32_t ring_of(const ngap_run_opts &o);
 // Runs tasks on ctx (batched kernel), with the overflow fallback of o.
 int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> &tasks,

@@ -18,13 +18,6 @@
 
 #include "runtime.h"
 
-namespace ngapamd {
-hipError_t launch_reset(hipStream_t s, unsigned long long *rc, unsigned long long *items,
-                        uint32_t *head, uint32_t *ovf, uint32_t n, uint32_t *ovf_count,
-                        int keep_reports);
-hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
-}  // namespace ngapamd
-
 using namespace ngapamd;
 
 namespace {
@@ -198,7 +191,7 @@ int launch_slot(ngap_server *s, Slot &sl) {
   sl.blocks = std::max<uint32_t>(
       1, std::min<uint32_t>(s->blocks_per_launch, ((uint32_t)sl.tasks.size() + 3) / 4));
   HIP_TRY(launch_reset(sl.ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
-                       P.num_tasks, P.overflow_count, 0));
+                       P.num_tasks, P.overflow_count, 0, nullptr));
   HIP_TRY(launch_batched(sl.ctx.stream, P, sl.blocks, lds_bytes(P.ring_cap)));
   HIP_TRY(hipEventRecord(sl.ctx.ev1, sl.ctx.stream));
   sl.bytes = total + report_cap * 8 + spill_bytes;
@@ -289,7 +282,7 @@ int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
   uint32_t blocks = 0;
   if (!ring_of(*opts))
     return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
-  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts), false);
+  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts), 0);
   if (rc) return rc;
   // every launch gets the whole persistent grid: launches in flight share the
   // CUs, and a lone launch is not held to a fraction of the GPU

@@ -62,7 +62,7 @@ class Engine:
                 result_capacity=54619400, window=0, spill_capacity=0, persistent_blocks=0,
                 use_graph=False, data_buffer_fetch_size=128, add_aas_start=0,
                 add_aas_interval=1, tasks=None, fallback=False, cross_groups=1,
-                ring_capacity=0, task_traces=False):
+                ring_capacity=0, task_traces=False, root_pass=None, root_capacity=0):
         o = _lib.RunOpts()
         _lib.lib().ngap_default_opts(C.byref(o))
         if algorithm not in ALGORITHMS:
@@ -86,6 +86,9 @@ class Engine:
         o.cross_groups = int(cross_groups)
         o.ring_capacity = int(ring_capacity)
         o.task_traces = int(bool(task_traces))
+        if root_pass is not None:  # None keeps the library default
+            o.root_pass = int(bool(root_pass))
+        o.root_capacity = int(root_capacity)
         if tasks is not None:
             # group -1 (or ALL_GROUPS) = a work item for every group
             arr = np.ascontiguousarray(np.asarray(tasks, np.int64).astype(np.uint32)

@@ -155,6 +155,41 @@ def test_ring_capacity_changes_nothing_but_speed(engine):
         engine.run(algorithm="nonblockingallgroups", schedule="batched", ring_capacity=96)
 
 
+def test_root_pass_changes_nothing_but_speed(engine):
+    groups, data = _workload(seed=27)
+    L = 1 << 13
+    want, _ = oracle.run(groups, data, L)
+    engine.load_groups(groups, 3)
+    engine.set_input(data, L)
+    for depth in (1, 2, 3):
+        for x in (0, 1):
+            for cap in (0, 1):
+                say("root pass depth", depth, "cross", x, "capacity", cap)
+                engine.run(algorithm="nonblockingallgroups", schedule="batched",
+                           precompute_depth=depth, cross_groups=x, root_pass=True,
+                           root_capacity=cap, unique=True, window=1500)
+                assert np.array_equal(engine.reports(), want)
+    # graph replays of a root-pass run
+    opts = engine.options(algorithm="nonblockingallgroups", schedule="batched",
+                          precompute_depth=3, root_pass=True, use_graph=True)
+    for rep in range(3):
+        say("root pass graph replay", rep)
+        engine.run(opts)
+        assert np.array_equal(engine.reports(), want)
+        engine.run(algorithm="nonblockinggroups", schedule="plain")
+    # root areas far too small (16 all-input states match every byte): the
+    # work items are re-run without a root pass even with the fallback off
+    lg, ldata, _ = synth.literal64(input_len=1 << 16)
+    lwant, _ = oracle.run(lg, ldata, 1 << 14)
+    engine.load_groups(lg, 1)
+    engine.set_input(ldata, 1 << 14)
+    say("root areas overflow")
+    st = engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=1,
+                    root_pass=True, root_capacity=1, fallback=False)
+    assert st["fallback_tasks"] > 0
+    assert np.array_equal(engine.reports(), lwant)
+
+
 def test_autotuner_picks_knobs_not_results(engine):
     groups, data = _workload(seed=24)
     L = 1 << 13
