@@ -24,8 +24,7 @@ struct DevImage {
   const uint32_t *sel[kMaxMemoDepth];  // rank tables of depths 2 and 3
   const uint32_t *mask1;     // 16 words per group (layout.h)
   // cross-group root index (layout.h)
-  const uint32_t *xr1, *xe1, *xmask1, *xr2, *xe2, *xsod, *xs3, *xe3, *x2rep;
-  const int32_t *xl3;
+  const uint32_t *xr1, *xe1, *xmask1, *xr2, *xe2, *xsod, *xs3, *xp3, *x2rep;
   uint32_t n_xsod;
 };
 

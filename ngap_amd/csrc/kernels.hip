@@ -395,6 +395,35 @@ NGAP_DEV void for_entries(const WaveState &W, uint32_t e0, uint32_t ecnt, F &&f)
   }
 }
 
+// for_entries over two ranges per lane, [a0, a0 + ac) then [b0, b0 + bc)
+// (the two halves of a 128-position batch): f(ok, entry index, owner lane,
+// range 0 or 1).
+template <class F>
+NGAP_DEV void for_entries2(const WaveState &W, uint32_t a0, uint32_t ac, uint32_t b0,
+                           uint32_t bc, F &&f) {
+  const uint32_t n = ac + bc;
+  const uint32_t incl = wave_incl_scan(n, W.lane);
+  const uint32_t total = __shfl(incl, 63, 64);
+  for (uint32_t base = 0; base < total; base += kWave) {
+    const uint32_t j = base + W.lane;
+    const bool ok = j < total;
+    uint32_t lo = 0, hi = 63;  // smallest lane o with incl[o] > j
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (__shfl(incl, (int)mid, 64) > j) hi = mid;
+      else lo = mid + 1;
+    }
+    const uint32_t off = j - (__shfl(incl, (int)lo, 64) - __shfl(n, (int)lo, 64));
+    const uint32_t oac = __shfl(ac, (int)lo, 64);
+    // both bases are read by every lane before the select: a shuffle inside
+    // a divergent branch reads nothing from lanes outside that branch
+    const uint32_t oa0 = __shfl(a0, (int)lo, 64), ob0 = __shfl(b0, (int)lo, 64);
+    const bool second = off >= oac;
+    f(ok, second ? ob0 + (off - oac) : oa0 + off, lo, second ? 1u : 0u);
+  }
+}
+
 // Per-lane state of one position in roots_cross.
 struct CrossLane {
   uint32_t p, k, b0, e0, ec, t0, tc, r0, rc;
@@ -512,30 +541,23 @@ NGAP_DEV void roots_cross(const KParams &P, WaveState &W, uint32_t p0, uint32_t 
     }
   }
   if (P.memo_depth < 3) return;
-  // depth 3 from the 3-gram slots: an inline state is emitted at once; other
-  // slots point at (begin in xl3, length | group << 22) entries
+  // depth 3 from the 3-gram slots: an inline state is emitted at once; the
+  // other slots' (tagged state, group) pairs of both halves are spread over
+  // the wave together, one 8-byte load per pair
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const bool inl = (L[h].tc & kSlotInline) != 0;
     emit_one<Stage>(P, W, inl, (int32_t)L[h].t0, false, L[h].p + 2, (L[h].tc & 1023u) << 22);
     if (inl) L[h].tc = 0;
   }
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const uint32_t p = L[h].p;
-    if (wballot(L[h].tc != 0) == 0) continue;
-    for_entries(W, L[h].t0, L[h].tc, [&](bool ok, uint32_t idx, uint32_t o) {
-      const uint32_t pos = __shfl(p, (int)o, 64);
-      uint32_t beg = 0, cnt = 0, tag = 0;
-      if (ok) {
-        const uint2 e = reinterpret_cast<const uint2 *>(P.img.xe3)[idx];
-        beg = e.x;
-        cnt = e.y & (uint32_t)kIdMask;
-        tag = (e.y >> 22) << 22;
-      }
-      flatten<Stage>(P, W, cnt, beg, pos + 2, tag, P.img.xl3);
-    });
-  }
+  if (wballot(L[0].tc != 0 || L[1].tc != 0) == 0) return;
+  // some lane has pairs, so pair 0 is a valid stand-in address
+  for_entries2(W, L[0].t0, L[0].tc, L[1].t0, L[1].tc,
+               [&](bool ok, uint32_t idx, uint32_t o, uint32_t h) {
+                 const uint2 e = reinterpret_cast<const uint2 *>(P.img.xp3)[ok ? idx : 0u];
+                 emit_one<Stage>(P, W, ok, (int32_t)e.x, false, p0 + h * kWave + o + 2,
+                                 e.y << 22);
+               });
 }
 
 template <bool Stage = false>

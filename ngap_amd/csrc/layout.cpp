@@ -15,7 +15,7 @@ uint64_t Image::bytes() const {
   for (auto &m : memo)
     b += (m.beg.size() + m.nrep.size() + m.sel.size() + m.list.size()) * 4;
   b += (mask1.size() + xr1.size() + xe1.size() + xmask1.size() + xr2.size() + xe2.size() +
-        xsod.size() + xs3.size() + xe3.size() + xl3.size() + x2rep.size()) * 4;
+        xsod.size() + xs3.size() + xp3.size() + x2rep.size()) * 4;
   return b;
 }
 
@@ -346,8 +346,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
       for (size_t i = 0; i < key3.size(); i++) order[at[key3[i]]++] = (uint32_t)i;
     }
     img.xs3.assign(2u << 24, 0);
-    img.xe3.reserve(key3.size() * 2);
-    img.xl3.reserve(m3.list.size());
+    img.xp3.reserve(m3.list.size() * 2);
     for (uint32_t k = 0; k < (1u << 24); k++) {
       const uint32_t n = start[k + 1] - start[k];
       if (n == 0) continue;
@@ -358,15 +357,20 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
         img.xs3[2 * (size_t)k + 1] = kSlotInline | grp3[li0];
         continue;
       }
-      img.xs3[2 * (size_t)k] = (uint32_t)(img.xe3.size() / 2);
-      img.xs3[2 * (size_t)k + 1] = n;
+      const size_t first = img.xp3.size() / 2;
       for (uint32_t j = start[k]; j < start[k + 1]; j++) {
         const uint32_t li = order[j];
-        img.xe3.push_back((uint32_t)img.xl3.size());
-        img.xe3.push_back((m3.beg[li + 1] - m3.beg[li]) | (grp3[li] << 22));
-        img.xl3.insert(img.xl3.end(), m3.list.begin() + m3.beg[li],
-                       m3.list.begin() + m3.beg[li + 1]);
+        for (uint32_t i = m3.beg[li]; i < m3.beg[li + 1]; i++) {
+          img.xp3.push_back((uint32_t)m3.list[i]);
+          img.xp3.push_back(grp3[li]);
+        }
       }
+      if (img.xp3.size() / 2 >= (1ull << 31)) {
+        err = "3-gram index too large (2^31 pairs); use a smaller precompute depth";
+        return false;
+      }
+      img.xs3[2 * (size_t)k] = (uint32_t)first;
+      img.xs3[2 * (size_t)k + 1] = (uint32_t)(img.xp3.size() / 2 - first);
     }
     // 2-grams with a reporting depth-2 list (their reports come from depth 2)
     img.x2rep.assign(2048, 0);

@@ -110,12 +110,12 @@ struct Image {
   //       8-byte slot xs3[2k], xs3[2k+1]: (0, 0) when no group has one;
   //       (tagged state, kSlotInline | group) when exactly one group has one
   //       and it holds one state (the common case: one load, no list);
-  //       otherwise (first entry, entry count) into xe3, whose entries are
-  //       (begin in xl3, length | group << 22) with the lists copied into xl3
-  //       in key order.  x2rep marks the 2-grams that have a reporting
-  //       depth-2 list (2048 words).
-  std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod, xs3, xe3, x2rep;
-  std::vector<int32_t> xl3;
+  //       otherwise (first pair, pair count) into xp3, whose 8-byte pairs
+  //       (tagged state, group) are the states of every group's list, group
+  //       by group in list order: a lookup reads the slot, then its pairs,
+  //       with no list indirection.  x2rep marks the 2-grams that have a
+  //       reporting depth-2 list (2048 words).
+  std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod, xs3, xp3, x2rep;
   // statistics (for the learned scheduler's features and reports)
   std::vector<uint32_t> stat_edges, stat_aas, stat_sod, stat_report,
       stat_wide, stat_selfloop;

@@ -44,6 +44,7 @@ RunCtx::~RunCtx() {
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
+  if (host_cnt) (void)hipHostFree(host_cnt);
 }
 
 int RunCtx::init() {
@@ -51,6 +52,16 @@ int RunCtx::init() {
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
   HIP_TRY(counters.reserve(128 + kQueues * kQueueStride * 4));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&host_cnt), 32, hipHostMallocDefault));
+  return NGAP_OK;
+}
+
+int RunCtx::finish(float *ms, unsigned long long cnt[4]) {
+  HIP_TRY(hipEventRecord(ev1, stream));
+  HIP_TRY(hipMemcpyAsync(host_cnt, counters.p, 32, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  HIP_TRY(hipEventElapsedTime(ms, ev0, ev1));
+  memcpy(cnt, host_cnt, 32);
   return NGAP_OK;
 }
 
@@ -133,8 +144,7 @@ KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_op
   P.img.xe2 = eng->d_xe2.as<uint32_t>();
   P.img.xsod = eng->d_xsod.as<uint32_t>();
   P.img.xs3 = eng->d_xs3.as<uint32_t>();
-  P.img.xe3 = eng->d_xe3.as<uint32_t>();
-  P.img.xl3 = eng->d_xl3.as<int32_t>();
+  P.img.xp3 = eng->d_xp3.as<uint32_t>();
   P.img.x2rep = eng->d_x2rep.as<uint32_t>();
   P.img.n_xsod = (uint32_t)(eng->img.xsod.size() / 2);
   P.num_groups = (uint32_t)eng->img.num_groups;
@@ -317,12 +327,10 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
   rc = launch_batched_ctx(ctx, P, blocks, T, false, o.use_graph != 0);
   if (rc) return rc;
-  HIP_TRY(hipEventRecord(ctx.ev1, ctx.stream));
-  HIP_TRY(hipEventSynchronize(ctx.ev1));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
   unsigned long long cnt[4] = {0, 0, 0, 0};
-  HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 32, hipMemcpyDeviceToHost));
+  rc = ctx.finish(&ms, cnt);
+  if (rc) return rc;
   std::vector<uint32_t> bad;
   bool spill_over = false;  // some work item lost items (not only its root area)
   if (cnt[3] && T) {  // per-item flags only when some work item overflowed
@@ -571,12 +579,10 @@ int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
                        P.overflow_count, 0, nullptr));
   if (blocking) HIP_TRY(launch_blocking(ctx.stream, P, units, lds_bytes()));
   else HIP_TRY(launch_plain(ctx.stream, P, units, lds_bytes()));
-  HIP_TRY(hipEventRecord(ctx.ev1, ctx.stream));
-  HIP_TRY(hipEventSynchronize(ctx.ev1));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
   unsigned long long cnt[4] = {0, 0, 0, 0};
-  HIP_TRY(hipMemcpy(cnt, ctx.counters.p, 32, hipMemcpyDeviceToHost));
+  const int frc = ctx.finish(&ms, cnt);
+  if (frc) return frc;
   const uint64_t overflowed = (uint32_t)cnt[3];
   ctx.last_count = cnt[0];
   ctx.last_report_off = o.report_off;
@@ -684,8 +690,7 @@ int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t nu
   HIP_TRY(eng->d_xe2.upload(im.xe2));
   HIP_TRY(eng->d_xsod.upload(im.xsod));
   HIP_TRY(eng->d_xs3.upload(im.xs3));
-  HIP_TRY(eng->d_xe3.upload(im.xe3));
-  HIP_TRY(eng->d_xl3.upload(im.xl3));
+  HIP_TRY(eng->d_xp3.upload(im.xp3));
   HIP_TRY(eng->d_x2rep.upload(im.x2rep));
   for (int d = 0; d < kMaxMemoDepth; d++) {
     HIP_TRY(eng->d_beg[d].upload(im.memo[d].beg));

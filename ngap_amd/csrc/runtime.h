@@ -52,6 +52,7 @@ struct DevBuf {
 struct RunCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  unsigned long long *host_cnt = nullptr;  // pinned: the run's counters after finish()
   DevBuf reports, counters, overflow, spill, tasks, task_items, fb_tasks, fb_spill;
   // root pass: pieces of the resident work items, each work item's area
   // offsets (items), the areas and the per-work-item counts
@@ -74,6 +75,10 @@ struct RunCtx {
   ~RunCtx();
   int init();
   void drop_graph();
+  // Ends a timed launch begun with ev0: records ev1, queues the copy of the
+  // four run counters (reports, items, -, overflowed units) into host_cnt
+  // behind it and waits for the stream once.  *ms = ev0..ev1.
+  int finish(float *ms, unsigned long long cnt[4]);
 };
 
 struct Mapping {
@@ -108,7 +113,7 @@ struct ngap_engine {
   ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;
   ngapamd::DevBuf d_beg[ngapamd::kMaxMemoDepth], d_nrep[ngapamd::kMaxMemoDepth],
       d_list[ngapamd::kMaxMemoDepth], d_sel[ngapamd::kMaxMemoDepth];
-  ngapamd::DevBuf d_xr1, d_xe1, d_xmask1, d_xr2, d_xe2, d_xsod, d_xs3, d_xe3, d_xl3, d_x2rep;
+  ngapamd::DevBuf d_xr1, d_xe1, d_xmask1, d_xr2, d_xe2, d_xsod, d_xs3, d_xp3, d_x2rep;
   ngapamd::DevBuf d_input;
   uint64_t chunk_len = 0;
   uint32_t num_seg = 0;

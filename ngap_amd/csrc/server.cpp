@@ -193,6 +193,10 @@ int launch_slot(ngap_server *s, Slot &sl) {
   HIP_TRY(launch_reset(sl.ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
                        P.num_tasks, P.overflow_count, 0, nullptr));
   HIP_TRY(launch_batched(sl.ctx.stream, P, sl.blocks, lds_bytes(P.ring_cap)));
+  // the counters follow the launch into pinned memory: retiring reads them
+  // without another copy
+  HIP_TRY(hipMemcpyAsync(sl.ctx.host_cnt, sl.ctx.counters.p, 32, hipMemcpyDeviceToHost,
+                         sl.ctx.stream));
   HIP_TRY(hipEventRecord(sl.ctx.ev1, sl.ctx.stream));
   sl.bytes = total + report_cap * 8 + spill_bytes;
   s->inflight_bytes += sl.bytes;
@@ -203,7 +207,7 @@ int launch_slot(ngap_server *s, Slot &sl) {
 int retire_slot(ngap_server *s, Slot &sl) {
   HIP_TRY(hipEventSynchronize(sl.ctx.ev1));
   unsigned long long c4[4] = {0, 0, 0, 0};
-  HIP_TRY(hipMemcpy(c4, sl.ctx.counters.p, 32, hipMemcpyDeviceToHost));
+  memcpy(c4, sl.ctx.host_cnt, 32);
   const unsigned long long cnt = c4[0];
   const bool redo = cnt > sl.ctx.report_cap || c4[3] != 0;
   if (redo) {

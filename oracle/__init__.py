@@ -106,13 +106,16 @@ def algorithmic_bytes(stats, unit_bytes):
 def algorithmic_bytes_cross(stats, positions, memo_depth):
     """Bytes of the all-group engine (memo depth >= 2), which reads the
     cross-group index of the deepest level once per position instead of
-    scanning every group: 1 B of input and an 8 B offset pair per position;
-    per non-empty deepest list its index entry (16 B at depth 2, 8 B at depth
-    3); then the same list entries, expanded items and reports as
+    scanning every group: 1 B of input and an 8 B index slot per position.
+    Depth 2: a 16 B index entry per non-empty list and 4 B per list entry;
+    depth 3: 8 B per list entry, a (state, group) pair read straight from the
+    3-gram slot's run.  Then the same expanded items and reports as
     algorithmic_bytes."""
-    per_list = 16 if memo_depth == 2 else 8
-    return (9 * positions + per_list * stats["deepest_lists"] + 4 * stats["memo_list_items"]
-            + 16 * stats["expanded_items"] + 4 * stats["expanded_tests"]
+    if memo_depth == 2:
+        roots = 16 * stats["deepest_lists"] + 4 * stats["memo_list_items"]
+    else:
+        roots = 8 * stats["memo_list_items"]
+    return (9 * positions + roots + 16 * stats["expanded_items"] + 4 * stats["expanded_tests"]
             + 8 * stats["reports"])
 
 

@@ -192,17 +192,24 @@ int main() {
                 if (want3 != std::set<int>{(int)(s0 & kIdMask)}) fail("inline slot");
               }
             } else {
+              // this group's pairs: one contiguous run holding its list
+              std::set<int> got;
+              uint32_t first = 0, last = 0, cnt = 0;
               for (uint32_t i = s0; i < s0 + s1; i++) {
-                if ((im.xe3[2 * i + 1] >> 22) != (uint32_t)gi) continue;
-                found++;
-                std::set<int> got;
-                const uint32_t beg = im.xe3[2 * i], cnt = im.xe3[2 * i + 1] & kIdMask;
-                for (uint32_t j = beg; j < beg + cnt; j++) got.insert(im.xl3[j] & kIdMask);
-                if (got != want3) fail("xl3 list");
-                if (s1 == 1 && cnt == 1) fail("a one-state slot not inline");
+                if (im.xp3[2 * (size_t)i + 1] != (uint32_t)gi) continue;
+                if (cnt == 0) first = i;
+                last = i;
+                cnt++;
+                got.insert(im.xp3[2 * (size_t)i] & kIdMask);
               }
+              if (cnt) {
+                found++;
+                if (got != want3 || got.size() != cnt) fail("xp3 pairs");
+                if (last - first + 1 != cnt) fail("xp3 run not contiguous");
+              }
+              if (s1 == 1) fail("a one-state slot not inline");
             }
-            if (found != (want3.empty() ? 0 : 1)) fail("xe3 entry");
+            if (found != (want3.empty() ? 0 : 1)) fail("xp3 group");
           }
         }
       }
