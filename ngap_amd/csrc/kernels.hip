@@ -448,7 +448,26 @@ NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, ui
     b1[h] = in[min_u(pc + 1, last)];
     b2[h] = in[min_u(pc + 2, last)];
   }
-  uint32_t x2w[2], x2a[2], x2b[2], x3a[2], x3b[2], x1a[2], x1b[2];
+  // Depth 3: the 3-gram slots first.  A slot flags the 3-grams whose first
+  // byte or first two bytes begin reporting lists of depth 1 or 2 in some
+  // group; the byte and 2-gram indexes are read only when some lane has such
+  // a 3-gram or fewer than three bytes left in its chunk (for every other
+  // lane their ranges are empty).
+  bool full = P.memo_depth < 3;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    L[h].t0 = L[h].tc = L[h].e0 = L[h].ec = L[h].r0 = L[h].rc = 0;
+    if (P.memo_depth >= 3) {
+      const uint32_t k = L[h].k;
+      const uint32_t key3 = (L[h].b0 << 16) | (b1[h] << 8) | b2[h];
+      const uint2 slot = reinterpret_cast<const uint2 *>(P.img.xs3)[key3];
+      L[h].t0 = slot.x;
+      L[h].tc = k >= 3 ? (slot.y & ~kSlotPrefixReports) : 0u;
+      full = full || (k >= 1 && k < 3) || (k >= 3 && (slot.y & kSlotPrefixReports));
+    }
+  }
+  if (wballot(full) == 0) return;
+  uint32_t x2w[2], x2a[2], x2b[2], x1a[2], x1b[2];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const uint32_t key2 = (L[h].b0 << 8) | b1[h];
@@ -456,15 +475,7 @@ NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, ui
     x2b[h] = P.img.xr2[key2 + 1];
     x1a[h] = P.img.xr1[L[h].b0];
     x1b[h] = P.img.xr1[L[h].b0 + 1];
-    x2w[h] = 0;
-    x3a[h] = x3b[h] = 0;
-    if (P.memo_depth >= 3) {
-      const uint32_t key3 = (key2 << 8) | b2[h];
-      x2w[h] = P.img.x2rep[key2 >> 5];
-      const uint2 slot = reinterpret_cast<const uint2 *>(P.img.xs3)[key3];
-      x3a[h] = slot.x;
-      x3b[h] = slot.y;
-    }
+    x2w[h] = P.memo_depth >= 3 ? P.img.x2rep[key2 >> 5] : 0u;
   }
 #pragma unroll
   for (int h = 0; h < 2; h++) {
@@ -474,9 +485,6 @@ NGAP_DEV void cross_lookup(const KParams &P, const WaveState &W, uint32_t p0, ui
     const bool need2 = k >= 2 && (P.memo_depth == 2 || k == 2 || ((x2w[h] >> (key2 & 31)) & 1u));
     L[h].e0 = x2a[h];
     L[h].ec = need2 ? x2b[h] - x2a[h] : 0u;
-    // the raw 3-gram slot (decoded in roots_cross)
-    L[h].t0 = x3a[h];
-    L[h].tc = P.memo_depth >= 3 && k >= 3 ? x3b[h] : 0u;
     L[h].r0 = x1a[h];
     L[h].rc = k >= 1 && ((W.mask1[8 + (b0 >> 5)] >> (b0 & 31)) & 1u) ? x1b[h] - x1a[h] : 0u;
   }

@@ -369,6 +369,10 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
         err = "3-gram index too large (2^31 pairs); use a smaller precompute depth";
         return false;
       }
+      if (img.xp3.size() / 2 - first >= kSlotPrefixReports) {
+        err = "a 3-gram has 2^30 depth-3 states or more; use a smaller precompute depth";
+        return false;
+      }
       img.xs3[2 * (size_t)k] = (uint32_t)first;
       img.xs3[2 * (size_t)k + 1] = (uint32_t)(img.xp3.size() / 2 - first);
     }
@@ -376,6 +380,12 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
     img.x2rep.assign(2048, 0);
     for (size_t i = 0; i < key2.size(); i++)
       if (img.memo[1].nrep[i]) img.x2rep[key2[i] >> 5] |= 1u << (key2[i] & 31);
+    // slots of the 3-grams whose prefixes report at depth 1 or 2
+    for (uint32_t k = 0; k < (1u << 24); k++) {
+      const uint32_t b0 = k >> 16, k2 = k >> 8;
+      if (((img.xmask1[b0 >> 5] >> (b0 & 31)) & 1u) || ((img.x2rep[k2 >> 5] >> (k2 & 31)) & 1u))
+        img.xs3[2 * (size_t)k + 1] |= kSlotPrefixReports;
+    }
   }
   for (auto &m : img.memo)
     if (m.list.size() >= (1ull << 31)) {

@@ -47,6 +47,9 @@ constexpr int kRecWords = 8;                  // int32 words per successor recor
 constexpr uint32_t kTestLiteral = 1u << 31;   // successor test: one byte (bits 0-7)
 constexpr uint32_t kTestAll = 1u << 30;       // successor test: every byte
 constexpr uint32_t kSlotInline = 1u << 31;    // 3-gram slot holds its one state
+// 3-gram slot flag: the first byte begins a reporting depth-1 list, or the
+// first two bytes a reporting depth-2 list, in some group
+constexpr uint32_t kSlotPrefixReports = 1u << 30;
 
 struct HostGroup {  // what the C ABI hands in for one group
   int32_t num_states = 0;
@@ -113,8 +116,12 @@ struct Image {
   //       otherwise (first pair, pair count) into xp3, whose 8-byte pairs
   //       (tagged state, group) are the states of every group's list, group
   //       by group in list order: a lookup reads the slot, then its pairs,
-  //       with no list indirection.  x2rep marks the 2-grams that have a
-  //       reporting depth-2 list (2048 words).
+  //       with no list indirection.  Bit 30 of the second word
+  //       (kSlotPrefixReports) flags the 3-grams whose first byte has a
+  //       reporting depth-1 list or whose first two bytes have a reporting
+  //       depth-2 list in some group (the count is the word without it).
+  //       x2rep marks the 2-grams that have a reporting depth-2 list (2048
+  //       words).
   std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod, xs3, xp3, x2rep;
   // statistics (for the learned scheduler's features and reports)
   std::vector<uint32_t> stat_edges, stat_aas, stat_sod, stat_report,

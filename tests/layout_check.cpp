@@ -184,7 +184,13 @@ int main() {
             // the 3-gram cross-group slot: this group's list, inline or
             // through its entry
             const uint32_t k3 = (key << 8) | (uint32_t)b2;
-            const uint32_t s0 = im.xs3[2 * (size_t)k3], s1 = im.xs3[2 * (size_t)k3 + 1];
+            const uint32_t s0 = im.xs3[2 * (size_t)k3], s1f = im.xs3[2 * (size_t)k3 + 1];
+            const uint32_t s1 = s1f & ~kSlotPrefixReports;
+            // the prefix flag: a reporting depth-1 list for b0 or depth-2
+            // list for (b0, b1) in some group
+            const bool pre = ((im.xmask1[b0 >> 5] >> (b0 & 31)) & 1u) ||
+                             ((im.x2rep[key >> 5] >> (key & 31)) & 1u);
+            if (pre != ((s1f & kSlotPrefixReports) != 0)) fail("3-gram prefix flag");
             int found = 0;
             if (s1 & kSlotInline) {
               if ((s1 & 1023u) == (uint32_t)gi) {
