@@ -119,6 +119,11 @@ typedef struct {
   uint32_t root_capacity;       /* root pass: root items reserved per input
                                    position of a work item, 0 = 4; a work item
                                    with more is re-run by the fallback */
+  uint64_t tasks_id;            /* batched, with tasks: the tasks_id a run over
+                                   these same work items returned in its stats.
+                                   The caller promises they are unchanged since
+                                   that run, and the engine skips comparing
+                                   them with its copy; 0 = compare */
 } ngap_run_opts;
 
 typedef struct {
@@ -130,6 +135,9 @@ typedef struct {
   uint32_t num_tasks;           /* batched work items */
   uint32_t fallback_tasks;      /* work items re-run after an overflow */
   double fallback_ms;           /* device time of those re-runs */
+  uint64_t tasks_id;            /* batched: id of the work items this run used
+                                   (for opts.tasks_id while they are unchanged;
+                                   loading groups or input retires it) */
 } ngap_run_stats;
 
 const char *ngap_last_error(void);

@@ -73,6 +73,7 @@ class RunOpts(C.Structure):
         ("task_traces", C.c_int32),
         ("root_pass", C.c_int32),
         ("root_capacity", C.c_uint32),
+        ("tasks_id", C.c_uint64),
     ]
 
 
@@ -86,6 +87,7 @@ class RunStats(C.Structure):
         ("num_tasks", C.c_uint32),
         ("fallback_tasks", C.c_uint32),
         ("fallback_ms", C.c_double),
+        ("tasks_id", C.c_uint64),
     ]
 
 

@@ -56,8 +56,8 @@ int RunCtx::init() {
   return NGAP_OK;
 }
 
-int RunCtx::finish(float *ms, unsigned long long cnt[4]) {
-  HIP_TRY(hipEventRecord(ev1, stream));
+int RunCtx::finish(float *ms, unsigned long long cnt[4], bool record_end) {
+  if (record_end) HIP_TRY(hipEventRecord(ev1, stream));
   HIP_TRY(hipMemcpyAsync(host_cnt, counters.p, 32, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   HIP_TRY(hipEventElapsedTime(ms, ev0, ev1));
@@ -231,41 +231,46 @@ static int root_areas(const ngap_engine *eng, RunCtx &ctx, const std::vector<Tas
   return NGAP_OK;
 }
 
-// One batched launch (counter reset + persistent kernel), replayed from a
-// captured graph when asked and when the parameters match the capture.  The
-// graph holds kernels only (the counters are cleared by k_reset).
+// One batched launch: clears the run's counters (k_reset, a kernel), then
+// launches the matching between the events `begin` and `end`: the root pass
+// (when on) and the persistent kernel, replayed from a captured graph when
+// use_graph is set and the parameters match the capture.  The graph holds
+// the matching kernels only; the events are recorded on the stream around
+// its launch (event records captured into a graph are not re-recorded on
+// replay), so begin..end is the matching plus the graph's launch latency.
 static int launch_batched_ctx(RunCtx &ctx, const KParams &P, uint32_t blocks, uint32_t n_flags,
-                              bool keep_reports, bool use_graph) {
+                              bool keep_reports, bool use_graph, hipEvent_t begin,
+                              hipEvent_t end) {
   const uint32_t lds = lds_bytes(P.ring_cap);
+  HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, n_flags,
+                       P.overflow_count, keep_reports, P.root_count));
   if (use_graph) {
-    std::vector<unsigned char> key(sizeof P + 12);
+    std::vector<unsigned char> key(sizeof P + 4);
     memcpy(key.data(), &P, sizeof P);
-    uint32_t extra[3] = {blocks, n_flags, keep_reports ? 1u : 0u};
-    memcpy(key.data() + sizeof P, extra, 12);
+    memcpy(key.data() + sizeof P, &blocks, 4);
     if (!ctx.gexec || key != ctx.gkey) {
       ctx.drop_graph();
       HIP_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-      hipError_t e1 = launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow,
-                                   n_flags, P.overflow_count, keep_reports, P.root_count);
       hipError_t e4 = P.root_items ? launch_roots(ctx.stream, P, lds) : hipSuccess;
       hipError_t e2 = launch_batched(ctx.stream, P, blocks, lds);
       hipGraph_t g = nullptr;
       hipError_t e3 = hipStreamEndCapture(ctx.stream, &g);
-      HIP_TRY(e1);
+      ctx.graph = g;
       HIP_TRY(e4);
       HIP_TRY(e2);
       HIP_TRY(e3);
-      ctx.graph = g;
       HIP_TRY(hipGraphInstantiate(&ctx.gexec, ctx.graph, nullptr, nullptr, 0));
       ctx.gkey = key;
     }
+    HIP_TRY(hipEventRecord(begin, ctx.stream));
     HIP_TRY(hipGraphLaunch(ctx.gexec, ctx.stream));
+    HIP_TRY(hipEventRecord(end, ctx.stream));
     return NGAP_OK;
   }
-  HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, n_flags,
-                       P.overflow_count, keep_reports, P.root_count));
+  HIP_TRY(hipEventRecord(begin, ctx.stream));
   if (P.root_items) HIP_TRY(launch_roots(ctx.stream, P, lds));
   HIP_TRY(launch_batched(ctx.stream, P, blocks, lds));
+  HIP_TRY(hipEventRecord(end, ctx.stream));
   return NGAP_OK;
 }
 
@@ -324,12 +329,11 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   P.spill = ctx.spill.as<uint64_t>();
   attach_ctx(P, ctx);
 
-  HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
-  rc = launch_batched_ctx(ctx, P, blocks, T, false, o.use_graph != 0);
+  rc = launch_batched_ctx(ctx, P, blocks, T, false, o.use_graph != 0, ctx.ev0, ctx.ev1);
   if (rc) return rc;
   float ms = 0;
   unsigned long long cnt[4] = {0, 0, 0, 0};
-  rc = ctx.finish(&ms, cnt);
+  rc = ctx.finish(&ms, cnt, false);
   if (rc) return rc;
   std::vector<uint32_t> bad;
   bool spill_over = false;  // some work item lost items (not only its root area)
@@ -383,10 +387,8 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
     F.root_off = nullptr;
     F.root_items = nullptr;
     F.root_count = nullptr;
-    HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
-    rc = launch_batched_ctx(ctx, F, fb_blocks, F.num_tasks, true, false);
+    rc = launch_batched_ctx(ctx, F, fb_blocks, F.num_tasks, true, false, ctx.ev0, ctx.ev1);
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(ctx.ev1, ctx.stream));
     HIP_TRY(hipEventSynchronize(ctx.ev1));
     HIP_TRY(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
     fb_ms += ms;
@@ -574,14 +576,14 @@ int run_plain(ngap_engine *eng, RunCtx &ctx, KParams P, const ngap_run_opts &o,
   HIP_TRY(ctx.overflow.reserve((size_t)units * 4));
   P.overflow = ctx.overflow.as<uint32_t>();
   attach_ctx(P, ctx);
-  HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));
   HIP_TRY(launch_reset(ctx.stream, P.report_count, P.items, P.task_head, P.overflow, units,
                        P.overflow_count, 0, nullptr));
+  HIP_TRY(hipEventRecord(ctx.ev0, ctx.stream));  // times the matching kernel alone
   if (blocking) HIP_TRY(launch_blocking(ctx.stream, P, units, lds_bytes()));
   else HIP_TRY(launch_plain(ctx.stream, P, units, lds_bytes()));
   float ms = 0;
   unsigned long long cnt[4] = {0, 0, 0, 0};
-  const int frc = ctx.finish(&ms, cnt);
+  const int frc = ctx.finish(&ms, cnt, true);
   if (frc) return frc;
   const uint64_t overflowed = (uint32_t)cnt[3];
   ctx.last_count = cnt[0];
@@ -774,7 +776,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
              sc.is_dup == (o.is_dup != 0);
   if (hit && kind == 0)
     hit = sc.memo_depth == mp.memo_depth && sc.host.size() == o.num_tasks &&
-          (o.num_tasks == 0 ||
+          (o.num_tasks == 0 || (o.tasks_id && o.tasks_id == sc.id) ||
            memcmp(sc.host.data(), o.tasks, o.num_tasks * sizeof(ngap_task)) == 0);
   else if (hit)
     hit = sc.window == o.window && sc.waves == waves;
@@ -797,7 +799,9 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
     sc.waves = waves;
     sc.id = eng->next_sched_id++;
   }
-  return run_tasks(eng, ctx, P, sc.tasks, sc.id, o, st);
+  const int rc = run_tasks(eng, ctx, P, sc.tasks, sc.id, o, st);
+  if (st) st->tasks_id = sc.id;
+  return rc;
 }
 
 int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,

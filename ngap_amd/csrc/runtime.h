@@ -75,10 +75,11 @@ struct RunCtx {
   ~RunCtx();
   int init();
   void drop_graph();
-  // Ends a timed launch begun with ev0: records ev1, queues the copy of the
-  // four run counters (reports, items, -, overflowed units) into host_cnt
-  // behind it and waits for the stream once.  *ms = ev0..ev1.
-  int finish(float *ms, unsigned long long cnt[4]);
+  // Ends a timed launch begun with ev0: records ev1 (unless the launch
+  // recorded it), queues the copy of the four run counters (reports, items,
+  // -, overflowed units) into host_cnt behind it and waits for the stream
+  // once.  *ms = ev0..ev1.
+  int finish(float *ms, unsigned long long cnt[4], bool record_end);
 };
 
 struct Mapping {

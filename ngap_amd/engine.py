@@ -121,10 +121,15 @@ class Engine:
             opts = self.options(**kw)
         st = _lib.RunStats()
         code = _lib.lib().ngap_run(self._h, C.byref(opts), C.byref(st))
+        if code == _lib.NGAP_OK and opts.tasks:
+            # the options own a private copy of their work items, so later runs
+            # with them may skip the engine's comparison of the work items
+            opts.tasks_id = st.tasks_id
         stats = dict(kernel_ms=st.kernel_ms, reports_raw=st.reports_raw,
                      overflow_units=st.overflow_units, items=st.items,
                      num_units=st.num_units, num_tasks=st.num_tasks,
-                     fallback_tasks=st.fallback_tasks, fallback_ms=st.fallback_ms)
+                     fallback_tasks=st.fallback_tasks, fallback_ms=st.fallback_ms,
+                     tasks_id=st.tasks_id)
         if code != _lib.NGAP_OK:
             err = NgapError(code, _lib.lib().ngap_last_error().decode(errors="replace"))
             err.stats = stats

@@ -31,14 +31,16 @@ def test_graph_replays_match_eager(engine):
     eager = dict(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
                  unique=True, window=2048)
     say("eager")
-    engine.run(**eager)
+    t_eager = min(engine.run(**eager)["kernel_ms"] for _ in range(3))
     assert np.array_equal(engine.reports(), want)
     opts = engine.options(use_graph=True, **eager)
     for rep in range(4):
         say("graph replay", rep)
-        engine.run(opts)
+        t_graph = engine.run(opts)["kernel_ms"]
         assert np.array_equal(engine.reports(), want)
-        # eager work between replays
+        # each replay is timed itself: not a stale event from an earlier launch
+        assert 0 < t_graph < 4 * t_eager + 0.5, (t_graph, t_eager)
+        # eager work between replays (a slower engine: a stale time would show)
         engine.run(algorithm="nonblockinggroups", schedule="plain")
         assert np.array_equal(engine.reports(), want)
 
@@ -72,6 +74,20 @@ def test_host_work_items(engine):
     assert np.array_equal(engine.reports(), want)
     items = engine.task_items()
     assert items.shape == (len(tasks),) and int(items.sum()) == st["items"]
+    # the same options again: the run returns a schedule id that the next
+    # run passes back (no comparison of the work items), with equal reports
+    say("schedule id reuse")
+    opts = engine.options(algorithm="nonblockingallgroups", schedule="batched",
+                          precompute_depth=3, tasks=tasks)
+    first = engine.run(opts)["tasks_id"]
+    assert first != 0 and opts.tasks_id == first
+    for _ in range(2):
+        assert engine.run(opts)["tasks_id"] == first
+        assert np.array_equal(engine.reports(), want)
+    # new input retires the id: the next run compares and gets a new one
+    engine.set_input(data, L)
+    assert engine.run(opts)["tasks_id"] != first
+    assert np.array_equal(engine.reports(), want)
     # a gap or an overlap is rejected before anything runs
     say("bad tilings")
     with pytest.raises(NgapError):
