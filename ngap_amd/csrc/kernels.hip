@@ -760,7 +760,9 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
       refill_from_spill(P, W);
       qn = W.tail - W.head;
     }
-    if (qn < kWave && W.spill_n == 0 && roots_left) {
+    // the next root batch runs while half the ring is still queued, so the
+    // expansion steps after it find items for every idle lane
+    if (qn < P.ring_cap / 2 && W.spill_n == 0 && roots_left) {
       uint32_t a, b;
       if (!src.next(W, a, b)) roots_left = false;
       else if (Cross) roots_cross(P, W, a, b);
@@ -914,7 +916,7 @@ NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
       refill_from_spill(P, W);
       qn = W.tail - W.head;
     }
-    if (qn < kWave && W.spill_n == 0 && more) {
+    if (qn < P.ring_cap / 2 && W.spill_n == 0 && more) {  // as in drain()
       uint32_t a, b;
       const int st = src.template next<RootItems>(P, W, a, b);
       if (st == 0) {
