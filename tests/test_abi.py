@@ -49,3 +49,28 @@ def test_null_arguments_are_errors():
     assert L.ngap_run(None, None, None) == _lib.NGAP_ERR_INVALID
     assert L.ngap_frontend_load(None, 1, -1, 1, 0, None) == _lib.NGAP_ERR_INVALID
     assert L.ngap_frontend_free(None) == _lib.NGAP_OK
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Size and every field offset of the ctypes mirrors equal what a C
+    compiler lays out from include/ngap_amd.h."""
+    pairs = [("ngap_task", _lib.Task), ("ngap_group_desc", _lib.GroupDesc),
+             ("ngap_run_opts", _lib.RunOpts), ("ngap_run_stats", _lib.RunStats)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ngap_amd.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe),
+                    str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out.splitlines()}
+    for cname, py in pairs:
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
