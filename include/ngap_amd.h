@@ -137,7 +137,8 @@ typedef struct {
   double fallback_ms;           /* device time of those re-runs */
   uint64_t tasks_id;            /* batched: id of the work items this run used
                                    (for opts.tasks_id while they are unchanged;
-                                   loading groups or input retires it) */
+                                   loading groups or input retires it; ids are
+                                   unique across the engines of a process) */
 } ngap_run_stats;
 
 const char *ngap_last_error(void);

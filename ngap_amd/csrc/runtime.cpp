@@ -12,8 +12,16 @@
 #include "runtime.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
+
+// Schedule ids are unique across the engines of a process, so an id handed
+// back through ngap_run_opts.tasks_id never matches another engine's schedule.
+static uint64_t next_schedule_id() {
+  static std::atomic<uint64_t> next{1};
+  return next.fetch_add(1);
+}
 
 int ngap_fail(int code, const std::string &msg) {
   ngap_set_last_error_internal(msg);
@@ -797,7 +805,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
     sc.memo_depth = mp.memo_depth;
     sc.window = o.window;
     sc.waves = waves;
-    sc.id = eng->next_sched_id++;
+    sc.id = next_schedule_id();
   }
   const int rc = run_tasks(eng, ctx, P, sc.tasks, sc.id, o, st);
   if (st) st->tasks_id = sc.id;

@@ -132,7 +132,6 @@ struct ngap_engine {
     std::vector<ngapamd::Task> tasks;
   } sched;
   uint64_t generation = 1;      // bumped by ngap_load_groups / ngap_set_input
-  uint64_t next_sched_id = 1;
 };
 
 namespace ngapamd {
