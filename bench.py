@@ -251,6 +251,9 @@ def serving_bench(rates, n_requests=128):
     for _, g, d in base:
         rid = srv.submit(g, d)
         alone.append(srv.drain([rid])[rid][0])
+    # one unmeasured pass with everything queued, so every in-flight slot has
+    # grown its device buffers before the sweep (steady-state serving)
+    run_poisson(srv, [(0.0, g, d) for _, g, d in base])
     out = {"groups": len(groups), "requests": n_requests,
            "input_bytes": int(sum(len(d) for _, _, d in base)), "rates": []}
     for rate in rates:
