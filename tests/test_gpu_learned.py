@@ -92,6 +92,9 @@ def test_learned_schedule_end_to_end(engine):
     waves = engine.batched_waves()
     assert waves > 0
     tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves=waves)
+    # the predicted peak sizes the worklist spill area below the static
+    # default of 65,536 items per wave (the memory the network saves)
+    assert learned.spill_capacity(pred, tcost) < 65536
     want, _ = oracle.run(groups, test, chunk)
     say("learned schedule run")
     st = engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2,
