@@ -58,3 +58,34 @@ def test_regex20k_full_size_invariants(engine):
     say("regex20k sampled chunks vs oracle")
     engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=2)
     assert np.array_equal(engine.reports(), want)
+
+
+def test_learned_workload_full_size_schedules_agree(engine):
+    """1,024 automata (log-uniform 50..50,000 states, a fifth with a leading
+    dot-star) over 8 MiB of drifting-Zipf input with planted pattern instances,
+    64 KiB chunks: the plain engine, the batched engine with all-group work
+    items at depth 3 and with per-group work items at depth 2 give the same
+    report set (about a thousand reports); truncated copies of sampled chunks
+    match the oracle."""
+    groups, data, chunk = synth.learned()
+    engine.load_groups(groups, 3)
+    engine.set_input(data, chunk)
+    base = None
+    for cfg in [dict(schedule="plain", precompute_depth=2),
+                dict(schedule="batched", precompute_depth=3, use_graph=True),
+                dict(schedule="batched", precompute_depth=2, cross_groups=0)]:
+        say("learned full", cfg)
+        engine.run(algorithm="nonblockingallgroups", unique=True, **cfg)
+        got = engine.reports()
+        if base is None:
+            base = got
+            assert got.size > 500
+        assert np.array_equal(got, base)
+    picks = [0, 64, 127]
+    L = 8192
+    sample = np.concatenate([data[c * chunk: c * chunk + L] for c in picks])
+    want, _ = oracle.run(groups, sample, L, len(picks))
+    engine.set_input(sample, L)
+    say("learned sampled chunks vs oracle", want.size)
+    engine.run(algorithm="nonblockingallgroups", schedule="batched", precompute_depth=3)
+    assert np.array_equal(engine.reports(), want)
