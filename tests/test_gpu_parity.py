@@ -255,3 +255,49 @@ def test_a_thousand_and_twenty_four_groups(engine):
             for s, d, u in (("plain", 2, 1), ("batched", 2, 0), ("batched", 3, 1))]
     want = check_all(engine, groups, data, 1 << 11, None, cfgs)
     assert int(np.max(want >> np.uint64(54))) == 1023 or want.size == 0
+
+
+def _top_id_group(n, pattern):
+    """A group of n states whose only edges form literal chains of `pattern`
+    placed at the top of the id range (the rest are isolated states): its
+    reports carry state ids up to n - 1."""
+    from ngap_amd.groups import Group
+    L = len(pattern)
+    first = n - L
+    row = np.zeros(n + 1, np.int32)
+    row[first + 1:] = np.minimum(np.arange(1, n - first + 1), L - 1)  # edges i -> i + 1 in the chain
+    col = np.arange(first + 1, n, dtype=np.int32)
+    sets = np.zeros(n * 8, np.uint32)
+    attrs = np.zeros(n, np.uint8)
+    for i, b in enumerate(pattern):
+        v = first + i
+        sets[v * 8 + b // 32] |= np.uint32(1 << (b % 32))
+    attrs[first] |= 1          # all-input start
+    attrs[n - 1] |= 2          # reports
+    return Group(row=row, col=col, sets=sets, attrs=attrs,
+                 aas=np.array([first], np.int32), sod=np.zeros(0, np.int32))
+
+
+def test_state_ids_at_the_22_bit_limit(engine):
+    """Group 1 has 2^22 states with its reporting chain at the top ids: the
+    report encoding (state | group << 22) carries state 4,194,303 of group 1
+    exactly; a group of 2^22 + 1 states is rejected before anything runs."""
+    n = 1 << 22
+    g0 = _top_id_group(64, b"abc")
+    g1 = _top_id_group(n, b"abcd")
+    rng = np.random.default_rng(5)
+    data = rng.choice(np.frombuffer(b"abcd", np.uint8), 1 << 14)
+    want, _ = oracle.run([g0, g1], data, 1 << 12)
+    grp = want >> np.uint64(54)
+    st = (want >> np.uint64(32)) & np.uint64((1 << 22) - 1)
+    assert want.size > 10 and int(st.max()) == n - 1 and set(grp.tolist()) == {0, 1}
+    engine.load_groups([g0, g1], 3)
+    engine.set_input(data, 1 << 12)
+    for sched, d in (("plain", 1), ("batched", 2), ("batched", 3)):
+        say("22-bit ids", sched, d)
+        engine.run(algorithm="nonblockingallgroups", schedule=sched, precompute_depth=d,
+                   unique=True)
+        assert np.array_equal(engine.reports(), want)
+    say("22-bit ids: one state too many")
+    with pytest.raises(NgapError):
+        engine.load_groups([g0, _top_id_group(n + 1, b"abcd")], 1)
