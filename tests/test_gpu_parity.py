@@ -192,6 +192,23 @@ def test_overflow_is_reported_not_wrong(engine):
     assert engine.reports().size == 4096
 
 
+def test_report_off_counts_without_writing(engine):
+    # --report-off: the reports are counted, not written; with one literal
+    # all-input state (no duplicates possible) the count is exact
+    g = make_group(1, [], [[ord("a")]], all_input=[0], reporting=[0])
+    engine.load_groups([g], 2)
+    engine.set_input(b"ab" * 2048, 4096, 1)
+    for sched in ("plain", "batched"):
+        say("report-off", sched)
+        st = engine.run(algorithm="nonblockingallgroups", schedule=sched, precompute_depth=2,
+                        report_off=True, result_capacity=10)
+        assert st["reports_raw"] == 2048
+        with pytest.raises(NgapError):   # nothing to read back after --report-off
+            engine.reports()
+        engine.run(algorithm="nonblockingallgroups", schedule=sched, precompute_depth=2)
+        assert engine.reports().size == 2048
+
+
 def test_spill_overflow_is_reported(engine):
     # a 4-ary tree of 'a' states three levels deep below an all-input root:
     # every position creates 4 + 16 + 64 items, far beyond a 512-item ring when
