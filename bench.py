@@ -69,44 +69,95 @@ def workload(name):
     return groups, data, train, chunk
 
 
-def sample_stats(groups, data, chunk, depth, n_units=4, sample_len=16384, threads=0):
-    """Oracle work counts on the first sample_len bytes of n_units chunks,
-    scaled to bytes per input byte (chunk-start effects are negligible)."""
+def cpu_info():
+    """Host CPU model, online CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    return model, nproc, usable
+
+
+def host_threads(units):
+    """Oracle threads: min(units, usable CPUs, 16).  The reference gives every
+    (group, chunk) unit a thread (ngap.cu:2190); 16 is the host CPU share of
+    one GPU on the measuring machine."""
+    _, _, usable = cpu_info()
+    return max(1, min(units, usable, 16))
+
+
+def spread_chunks(S, n):
+    return sorted(set(np.linspace(0, S - 1, min(n, S)).round().astype(int).tolist()))
+
+
+def sample_stats(groups, data, chunk, depth, max_chunks=32):
+    """Oracle work counts over whole chunks (the whole input, or max_chunks
+    chunks spread over it, at least 1 MiB for the benchmark workloads), with
+    the all-input states implicit (same counts as the reference loop, much
+    faster).  Returns the byte models per input byte and the sample."""
     import oracle
     S = len(data) // chunk
-    picks = np.linspace(0, S - 1, min(n_units, S)).astype(int)
-    sample = np.concatenate([data[c * chunk: c * chunk + min(sample_len, chunk)] for c in picks])
-    L = min(sample_len, chunk)
+    G = len(groups)
+    picks = spread_chunks(S, max_chunks)
+    mask = np.zeros(G * S, np.uint8)
+    for c in picks:
+        mask[c * G:(c + 1) * G] = 1
+    threads = host_threads(G * len(picks))
     t = time.time()
-    _, st = oracle.run(groups, sample, L, len(picks), threads=threads, memo_depth=depth)
+    rep, st = oracle.run(groups, data, chunk, S, unit_mask=mask, threads=threads,
+                         memo_depth=depth, implicit_aas=True)
     dt = time.time() - t
-    unit_bytes = sample.size * len(groups)
-    per_byte = oracle.algorithmic_bytes(st, unit_bytes) / sample.size
-    per_byte_ref = oracle.algorithmic_bytes_reference_model(st, unit_bytes) / sample.size
-    per_byte_cross = oracle.algorithmic_bytes_cross(st, sample.size, depth) / sample.size
-    return per_byte, per_byte_ref, per_byte_cross, st, sample.size, dt, len(picks)
+    positions = len(picks) * chunk
+    unit_bytes = positions * G
+    return dict(
+        per_group=oracle.algorithmic_bytes(st, unit_bytes) / positions,
+        reference=oracle.algorithmic_bytes_reference_model(st, unit_bytes) / positions,
+        cross=oracle.algorithmic_bytes_cross(st, positions, depth) / positions,
+        # engine-independent: the input once and each distinct report once
+        floor=(positions + 8 * st["reports"]) / positions,
+        stats=st, positions=positions, chunks=len(picks), seconds=dt, threads=threads,
+        reports=int(rep.size))
 
 
-def cpu_baseline(groups, data, chunk, budget_s=15.0):
-    """The oracle (a port of the reference's CPU validator) timed on host
-    cores over a bounded sample: one unit per core, each a slice of a chunk."""
+def cpu_baseline(groups, data, chunk, depth, budget_s=20.0):
+    """The oracle running the reference's CPU validator loop as written
+    (every all-input state tested at every position, ngap.cu:2171-2268) on
+    host threads over a bounded sample of whole-chunk prefixes: one unit per
+    (group, chunk), threads = min(units, usable CPUs, 16)."""
     import oracle
-    cores = min(16, os.cpu_count() or 1)
-    # calibrate on 2 KiB of one unit
+    model, nproc, usable = cpu_info()
+    S = len(data) // chunk
+    G = len(groups)
+    # calibrate on 2 KiB of one chunk, one thread
     L0 = min(2048, chunk)
     t = time.time()
-    oracle.run(groups, data[:L0], L0, 1, threads=1)
-    rate = L0 / max(1e-6, time.time() - t)  # bytes/s per core
-    L = int(min(chunk, max(256, rate * budget_s)))
-    S = len(data) // chunk
-    units = min(cores, S)
-    sample = np.concatenate([data[c * chunk: c * chunk + L] for c in range(units)])
+    oracle.run(groups, data[:L0], L0, 1, threads=1, implicit_aas=False, memo_depth=depth)
+    rate = L0 / max(1e-6, time.time() - t)  # input bytes/s of one thread over all groups
+    picks = spread_chunks(S, 16)
+    threads = host_threads(G * len(picks))
+    L = int(min(chunk, max(256, rate * threads * budget_s / len(picks))))
+    sample = np.concatenate([data[c * chunk: c * chunk + L] for c in picks])
+    units = G * len(picks)
+    threads = host_threads(units)
     t = time.time()
-    oracle.run(groups, sample, L, units, threads=units)
+    oracle.run(groups, sample, L, len(picks), threads=threads, implicit_aas=False,
+               memo_depth=depth)
     dt = time.time() - t
-    return dict(value=round(sample.size / dt / 1e6, 6), unit="MB/s", cores=units, kind="port",
-                sample=f"{units} units x {L} bytes (first bytes of chunks 0..{units - 1}), "
-                       f"{dt:.1f} s, oracle with unique=true semantics")
+    return dict(value=round(sample.size / dt / 1e6, 6), unit="MB/s", cores=threads,
+                kind="port", cpu_model=model, nproc=nproc, usable_cpus=usable,
+                threads_rule="min((group, chunk) units, usable CPUs, 16)",
+                sample=f"first {L} bytes of {len(picks)} chunks spread over the input "
+                       f"({sample.size} bytes x {G} groups = {units} units), {dt:.1f} s; "
+                       f"the reference's CPU validator loop with unique=true semantics")
 
 
 def copy_calibration(nbytes, reps):
@@ -153,11 +204,15 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
     pred = m.predict_pairs(hist, stats, all_pairs)
     waves = eng.batched_waves()
     cost = learned.predicted_cost(pred).reshape(len(groups), S * K)
-    if depth >= 2 and len(groups) > 1:   # all-group work items
+    # per-group work items, and all-group ones where the cross-group index
+    # exists (several groups, depth >= 2): the headline schedule
+    ptasks, pcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves)
+    if depth >= 2 and len(groups) > 1:
         tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, waves)
     else:
-        tasks, tcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves)
+        tasks, tcost = ptasks, pcost
     spill = learned.spill_capacity(pred, tcost)
+    pspill = learned.spill_capacity(pred, pcost)
     t_sched = time.time() - t1
     # prediction quality on the test input's own traces
     tp, ty = learned.collect_traces(eng, test, chunk, len(groups), depth)
@@ -171,12 +226,77 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
     opts = eng.options(schedule="batched", tasks=tasks, spill_capacity=spill, fallback=True,
                        use_graph=True, algorithm="nonblockingallgroups",
                        precompute_depth=depth, unique=True)
-    return opts, info
+    popts = None
+    if depth >= 2 and len(groups) > 1:
+        popts = eng.options(schedule="batched", tasks=ptasks, spill_capacity=pspill,
+                            fallback=True, use_graph=True, algorithm="nonblockingallgroups",
+                            precompute_depth=depth, unique=True)
+        info["pergroup_tasks"] = int(len(ptasks))
+    return opts, popts, info
 
 
-def compare(name, depth, steps, warmup, learned_on=True, tune=True):
-    """All engines on one workload; returns (result dict, per-engine runs,
-    groups, test input, chunk)."""
+def roofline_of(groups, data, chunk, depth, kern_ms, engine_label, workload, traffic_key):
+    """Roofline object of the dominant kernel (k_batched_stream) for one
+    workload: algorithmic bytes from the oracle's exact counts over whole
+    chunks divided by the kernel's HIP-event time; traffic from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary when one exists for this
+    workload and engine."""
+    ss = sample_stats(groups, data, chunk, depth)
+    cross = depth >= 2 and len(groups) > 1
+    model = ss["cross"] if cross else ss["per_group"]
+    total = (len(data) // chunk) * chunk
+    log(f"{workload} oracle sample: {ss['chunks']} whole chunks ({ss['positions']} bytes) in "
+        f"{ss['seconds']:.1f} s on {ss['threads']} threads: {model:.2f} algorithmic bytes per "
+        f"input byte ({ss['per_group']:.1f} group by group, {ss['reference']:.1f} for the "
+        f"reference algorithm, floor {ss['floor']:.4f}); {ss['stats']}")
+    achieved = model * total / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath)).get(traffic_key)
+        if t and t.get("engine") == engine_label and t.get("precompute_depth") == depth:
+            traffic = t["bytes_per_launch"]
+    return dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
+                traffic_gbs=round(traffic / (kern_ms * 1e-3) / 1e9, 1) if traffic else None,
+                kernel="k_batched_stream", kernel_ms=round(kern_ms, 4),
+                work_items="all-group" if cross else "per-group",
+                algorithmic_bytes_per_input_byte=round(model, 3),
+                floor_bytes_per_input_byte=round(ss["floor"], 5),
+                floor_frac=round(ss["floor"] * total / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+                per_group_model_bytes_per_input_byte=round(ss["per_group"], 3),
+                reference_model_bytes_per_input_byte=round(ss["reference"], 2),
+                sample=f"{ss['chunks']} whole chunks, {ss['positions']} bytes")
+
+
+def end_to_end(eng, data, chunk, opts, steps):
+    """Input upload + matching + report download, sort and unique per step
+    (wall clock), with the time of each part."""
+    t_in = t_run = t_out = 0.0
+    n = 0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        eng.set_input(data, chunk)
+        t1 = time.perf_counter()
+        eng.run(opts)
+        t2 = time.perf_counter()
+        n = eng.reports().size
+        t3 = time.perf_counter()
+        t_in += t1 - t0
+        t_run += t2 - t1
+        t_out += t3 - t2
+    total = (len(data) // chunk) * chunk
+    wall = (t_in + t_run + t_out) / steps
+    return dict(mbps=round(total / wall / 1e6, 3), ms_per_step=round(wall * 1e3, 4),
+                upload_ms=round(t_in / steps * 1e3, 4), run_ms=round(t_run / steps * 1e3, 4),
+                reports_ms=round(t_out / steps * 1e3, 4), reports=int(n))
+
+
+def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, check=True,
+            cpu=True):
+    """All engines on one workload, each timed `repeats` times (interleaved)
+    over `steps` steps; returns (result dict, per-engine runs, groups, input,
+    chunk)."""
     from ngap_amd import Engine
     t = time.time()
     groups, test, train, chunk = workload(name)
@@ -184,55 +304,104 @@ def compare(name, depth, steps, warmup, learned_on=True, tune=True):
     log(f"workload {name}: {len(groups)} groups, {sum(g.num_states for g in groups)} states, "
         f"{total} bytes, chunk {chunk} ({time.time() - t:.1f} s)")
     eng = Engine(0)
-    eng.load_groups(groups, max_precompute_depth=max(depth, 1))
+    # the image built by host threads (then uploaded), then by the device
+    # builder, which the runs use
+    load = {}
+    for where in ("host", "device"):
+        eng.load_groups(groups, max_precompute_depth=max(depth, 1), build=where)
+        iinfo = eng.image_info()
+        load[where] = round(iinfo["build_ms"], 1)
+    log(f"{name}: image of {iinfo['bytes'] / 1e6:.1f} MB built in {load['host']} ms on the host, "
+        f"{load['device']} ms on the device")
     common = dict(algorithm="nonblockingallgroups", precompute_depth=depth, unique=True)
-    runs = {}
-
-    def run(label, opts):
-        wall, kern, st = timed(eng, opts, steps, warmup)
-        runs[label] = dict(wall_ms=wall, kernel_ms=kern, stats=st, reports=eng.reports())
-        log(f"{name} {label}: {wall:.3f} ms/step wall, {kern:.3f} ms kernel, "
-            f"{total / wall / 1e3:.1f} MB/s, items {st['items']}, tasks {st['num_tasks']}, "
-            f"fallback {st['fallback_tasks']}, reports {runs[label]['reports'].size}")
-
     info = None
+    engines = {}
+    lpopts = None
     if learned_on:
-        lopts, info = learned_schedule(eng, groups, train, test, chunk, max(depth, 1))
+        lopts, lpopts, info = learned_schedule(eng, groups, train, test, chunk, max(depth, 1))
         log(f"{name} learned schedule: {info}")
     else:
         eng.set_input(test, chunk)
-    run("plain", eng.options(schedule="plain", **common))
-    run("static", eng.options(schedule="batched", use_graph=True, **common))
+    engines["plain"] = eng.options(schedule="plain", **common)
+    engines["static"] = eng.options(schedule="batched", use_graph=True, **common)
+    if len(groups) > 1 and depth >= 2:
+        # batched, one group per work item (no cross-group index)
+        engines["pergroup"] = eng.options(schedule="batched", use_graph=True, cross_groups=0,
+                                          **common)
+    tuned_knobs = None
     if tune:
         topts, tms = eng.autotune(max_runs=12, **common)
         topts.use_graph = 1
-        run("tuned", topts)
+        engines["tuned"] = topts
         tuned_knobs = dict(window=int(topts.window), persistent_blocks=int(topts.persistent_blocks),
-                           spill_capacity=int(topts.spill_capacity))
+                           spill_capacity=int(topts.spill_capacity),
+                           ring_capacity=int(topts.ring_capacity), root_pass=int(topts.root_pass))
     if learned_on:
-        run("learned", lopts)
+        engines["learned"] = lopts
+        if lpopts is not None:
+            # the learned schedule of per-group work items, next to "pergroup"
+            engines["learned_pergroup"] = lpopts
+    runs = {k: dict(walls=[], kerns=[]) for k in engines}
+    for rep in range(repeats):
+        for label, opts in engines.items():
+            if label == "plain" and rep > 0:
+                continue  # slow and steady: one repetition
+            wall, kern, st = timed(eng, opts, steps, warmup if rep == 0 else 1)
+            r = runs[label]
+            r["walls"].append(wall)
+            r["kerns"].append(kern)
+            r["stats"] = st
+            r["reports"] = eng.reports()
+    for label, r in runs.items():
+        r["wall_ms"] = float(np.median(r["walls"]))
+        r["kernel_ms"] = float(np.median(r["kerns"]))
+        st = r["stats"]
+        log(f"{name} {label}: {r['wall_ms']:.3f} ms/step wall (runs {['%.4f' % w for w in r['walls']]}), "
+            f"{r['kernel_ms']:.4f} ms kernel, {total / r['wall_ms'] / 1e3:.1f} MB/s, "
+            f"items {st['items']}, tasks {st['num_tasks']}, fallback {st['fallback_tasks']}, "
+            f"reports {r['reports'].size}")
+    main_s = "learned" if learned_on else "static"
+    e2e = end_to_end(eng, test, chunk, engines[main_s], max(3, steps // 2))
+    log(f"{name} end to end ({main_s}): {e2e}")
     eng.close()
     ref = runs["plain"]["reports"]
     res = {"workload": name, "groups": len(groups),
            "states": int(sum(g.num_states for g in groups)), "input_bytes": int(total),
-           "chunk": int(chunk), "reports": int(ref.size),
+           "chunk": int(chunk), "reports": int(ref.size), "load_groups_ms": load,
+           "image_bytes": int(iinfo["bytes"]),
            "bit_exact_vs_plain": all(bool(np.array_equal(r["reports"], ref))
-                                     for r in runs.values())}
+                                     for r in runs.values()),
+           "repeats": repeats}
     for k, r in runs.items():
         res[f"{k}_mbps"] = round(total / (r["wall_ms"] * 1e-3) / 1e6, 3)
         res[f"{k}_kernel_ms"] = round(r["kernel_ms"], 4)
+        if len(r["walls"]) > 1:
+            res[f"{k}_mbps_runs"] = [round(total / (w * 1e-3) / 1e6, 1) for w in r["walls"]]
         if k != "plain":
             res[f"speedup_{k}_vs_plain"] = round(runs["plain"]["wall_ms"] / r["wall_ms"], 3)
-    if tune:
+    if "learned" in runs and "static" in runs:
+        res["learned_vs_static"] = round(runs["static"]["wall_ms"] / runs["learned"]["wall_ms"], 4)
+    if "learned_pergroup" in runs and "pergroup" in runs:
+        res["learned_pergroup_vs_pergroup"] = round(
+            runs["pergroup"]["wall_ms"] / runs["learned_pergroup"]["wall_ms"], 4)
+    res["end_to_end"] = dict(e2e, engine=main_s,
+                             what="set_input (H2D) + run + reports (D2H, sort, unique)")
+    if tuned_knobs:
         res["tuned_knobs"] = tuned_knobs
     if info:
         st = runs["learned"]["stats"]
         res["learned"] = dict(info, fallback_tasks=int(st["fallback_tasks"]),
                               overflow_rate=st["fallback_tasks"] / max(1, st["num_tasks"]))
+    if check:
+        res["roofline"] = roofline_of(groups, test, chunk, max(1, depth),
+                                      runs[main_s]["kernel_ms"], main_s, name, name)
+    if cpu:
+        res["cpu_baseline"] = cpu_baseline(groups, test, chunk, max(1, depth))
+        log(f"{name} cpu baseline: {res['cpu_baseline']}")
     return res, runs, groups, test, chunk
 
 
-def serving_bench(rates, n_requests=128):
+def serving_bench(rates, n_requests=1024, depth=3):
     """Serving workload: 64 resident automata (1k..200k states), Poisson
     requests of 1 KiB..4 MiB Zipf(1.1) bytes; p50/p99 latency and aggregate
     MB/s per arrival rate; every request's reports checked against the same
@@ -241,11 +410,13 @@ def serving_bench(rates, n_requests=128):
     from ngap_amd.serve import Server, run_poisson
     t0 = time.time()
     groups = synth.serving_groups()
+    t1 = time.time()
     eng = Engine(0)
-    eng.load_groups(groups, 2)
-    log(f"serving: {sum(g.num_states for g in groups)} states loaded in {time.time() - t0:.1f} s")
+    eng.load_groups(groups, depth)
+    log(f"serving: {sum(g.num_states for g in groups)} states generated in {t1 - t0:.1f} s, "
+        f"loaded at depth {depth} in {time.time() - t1:.1f} s")
     srv = Server(eng, budget_bytes=200 << 30, max_inflight=4, algorithm="nonblockingallgroups",
-                 precompute_depth=2, unique=True)
+                 precompute_depth=depth, unique=True)
     base = synth.serving_requests(seed=2004, n=n_requests)
     alone = []
     for _, g, d in base:
@@ -254,7 +425,7 @@ def serving_bench(rates, n_requests=128):
     # one unmeasured pass with everything queued, so every in-flight slot has
     # grown its device buffers before the sweep (steady-state serving)
     run_poisson(srv, [(0.0, g, d) for _, g, d in base])
-    out = {"groups": len(groups), "requests": n_requests,
+    out = {"groups": len(groups), "requests": n_requests, "precompute_depth": depth,
            "input_bytes": int(sum(len(d) for _, _, d in base)), "rates": []}
     for rate in rates:
         if np.isfinite(rate):
@@ -270,6 +441,7 @@ def serving_bench(rates, n_requests=128):
                    bit_exact_vs_alone=exact)
         out["rates"].append(row)
         log(f"serving rate {rate}/s: {row}")
+    out["server"] = srv.stats()
     srv.close()
     eng.close()
     return out
@@ -300,7 +472,10 @@ def main():
                     help="--precompute-depth of both engines (the reference's best setting is 3)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned schedule")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
-    ap.add_argument("--also", default="batch1024",
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed repetitions of --steps steps per engine (median reported)")
+    ap.add_argument("--requests", type=int, default=1024, help="serving requests per sweep")
+    ap.add_argument("--also", default="batch1024,regex20k,literal64",
                     help="comma-separated further workloads compared the same way, or 'none'")
     ap.add_argument("--serving", default="100,1000,10000,inf",
                     help="arrival rates (per second, inf = all at once) for the serving run; "
@@ -319,38 +494,13 @@ def main():
         sys.exit(2)
 
     res, runs, groups, data, chunk = compare(a.workload, a.depth, a.steps, a.warmup,
-                                             learned_on=not a.no_learned, tune=not a.no_tune)
+                                             learned_on=not a.no_learned, tune=not a.no_tune,
+                                             repeats=a.repeats, check=not a.no_check,
+                                             cpu=not a.no_cpu_baseline)
     main_s = "learned" if "learned" in runs else "static"
     total = res["input_bytes"]
     wall = runs[main_s]["wall_ms"]
     kern = runs[main_s]["kernel_ms"]
-    roofline = None
-    if not a.no_check:
-        depth = max(1, a.depth)
-        per_byte, per_byte_ref, per_byte_cross, st, nbytes, dt, nunits = sample_stats(
-            groups, data, chunk, depth)
-        cross = depth >= 2 and len(groups) > 1
-        model = per_byte_cross if cross else per_byte
-        log(f"oracle sample: {nbytes} bytes over {nunits} chunks in {dt:.1f} s, "
-            f"{model:.2f} algorithmic bytes per input byte ({per_byte:.1f} group by group, "
-            f"{per_byte_ref:.1f} for the reference algorithm); {st}")
-        achieved = model * total / (kern * 1e-3) / 1e9
-        # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-        # on this workload and engine (profiles/), scaled by the copy calibration
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-        if os.path.exists(tpath):
-            t = json.load(open(tpath)).get(a.workload)
-            if t and t.get("engine") == main_s and t.get("precompute_depth") == a.depth:
-                traffic = t["bytes_per_launch"]
-        roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic,
-                        traffic_gbs=round(traffic / (kern * 1e-3) / 1e9, 1) if traffic else None,
-                        kernel="k_batched_stream",
-                        work_items="all-group" if cross else "per-group",
-                        algorithmic_bytes_per_input_byte=round(model, 3),
-                        per_group_model_bytes_per_input_byte=round(per_byte, 3),
-                        reference_model_bytes_per_input_byte=round(per_byte_ref, 2))
     out = {
         "metric": METRIC, "value": res[f"{main_s}_mbps"], "unit": "MB/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall, 4),
@@ -364,19 +514,21 @@ def main():
         "plain_mbps": res["plain_mbps"],
         "speedup_vs_plain": round(runs["plain"]["wall_ms"] / wall, 3),
         "bit_exact_vs_plain": res["bit_exact_vs_plain"],
+        "end_to_end": res["end_to_end"],
         "engines": res,
-        "roofline": roofline,
+        "roofline": res.get("roofline"),
     }
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(groups, data, chunk)
-        log(f"cpu baseline: {out['cpu_baseline']}")
+    if "cpu_baseline" in res:
+        out["cpu_baseline"] = res["cpu_baseline"]
     del runs, groups, data
     if a.also != "none":
         out["also"] = [compare(w, a.depth, a.steps, a.warmup, learned_on=not a.no_learned,
-                               tune=not a.no_tune)[0]
+                               tune=not a.no_tune, repeats=a.repeats, check=not a.no_check,
+                               cpu=not a.no_cpu_baseline)[0]
                        for w in a.also.split(",") if w and w != a.workload]
     if a.serving != "none":
-        out["serving"] = serving_bench([float(r) for r in a.serving.split(",")])
+        out["serving"] = serving_bench([float(r) for r in a.serving.split(",")],
+                                       n_requests=a.requests)
     print(json.dumps(out), flush=True)
 
 

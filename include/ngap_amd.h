@@ -147,9 +147,39 @@ void ngap_default_opts(ngap_run_opts *opts);
 int ngap_create(int32_t device, ngap_engine **out);
 int ngap_destroy(ngap_engine *eng);
 
-/* Builds the device automata image (memo tables up to max_precompute_depth). */
+/* Builds the device automata image (memo tables up to max_precompute_depth)
+ * on the device (HIP kernels over the uploaded CSR arrays). */
 int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups,
                      int32_t num_groups, int32_t max_precompute_depth);
+
+/* The same with the builder chosen: NGAP_BUILD_DEVICE (what ngap_load_groups
+ * does) or NGAP_BUILD_HOST (host threads, then one upload per array).  Both
+ * give the same arrays byte for byte.  The reference builds its memo tables
+ * on the host (code/src/ngap/ngap.cu:2358-2513,
+ * code/src/commons/precompute_table.cu:25-395). */
+#define NGAP_BUILD_DEVICE 0
+#define NGAP_BUILD_HOST 1
+int ngap_load_groups_ex(ngap_engine *eng, const ngap_group_desc *groups, int32_t num_groups,
+                        int32_t max_precompute_depth, int32_t build);
+
+/* The loaded image: where it was built, its size and the wall time of the
+ * load (build and upload). */
+typedef struct {
+  int32_t built_on_device;
+  int32_t memo_depth;
+  int32_t num_groups;
+  uint64_t num_states;
+  uint64_t bytes;
+  double build_ms;
+} ngap_image_summary;
+int ngap_image_info(ngap_engine *eng, ngap_image_summary *info);
+
+/* Copies the device image array `name` (meta, succ4, cls, sod, beg1-3,
+ * nrep1-3, list1-3, sel2, sel3, mask1, xr1, xe1, xmask1, xr2, xe2, xsod, xs3,
+ * xp3, x2rep; layout in ngap_amd/csrc/layout.h) into out (at most cap bytes);
+ * *n_bytes gets its size (out may be NULL to query it). */
+int ngap_image_array(ngap_engine *eng, const char *name, void *out, uint64_t cap,
+                     uint64_t *n_bytes);
 
 /* Uploads num_seg chunks of chunk_len bytes (the concatenated input). */
 int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
@@ -195,8 +225,9 @@ int ngap_get_tasks(ngap_engine *eng, ngap_task *out, uint64_t cap, uint64_t *n_o
  * requests that fit are packed into one launch on that slot's stream. */
 typedef struct ngap_server ngap_server;
 
-/* opts: algorithm / precompute depth / unique as for ngap_run.  budget_bytes
- * 0 = 200 GB. */
+/* opts: algorithm / precompute depth / unique as for ngap_run (report_off is
+ * rejected: requests return their reports).  budget_bytes 0 = 200 GB.  A
+ * request that alone needs more than the budget fails (state 3). */
 int ngap_server_create(ngap_engine *eng, uint64_t budget_bytes, int32_t max_inflight,
                        const ngap_run_opts *opts, ngap_server **out);
 int ngap_server_destroy(ngap_server *srv);
@@ -212,6 +243,24 @@ int ngap_server_poll(ngap_server *srv, int32_t block);
 int ngap_server_take(ngap_server *srv, uint64_t req_id, int32_t *state, uint64_t *out,
                      uint64_t cap, uint64_t *n_out, double *latency_ms);
 int ngap_server_pending(ngap_server *srv, uint64_t *queued, uint64_t *inflight);
+
+/* Memory and admission figures of a server.  held_bytes counts the device
+ * buffers of every launch slot (input, last-byte map, reports, work items,
+ * worklist spill), in flight or kept for reuse; admission keeps it within
+ * budget_bytes (idle slots give their buffers back when a launch needs the
+ * room).  A launch whose worklists or report buffer overflowed is re-run with
+ * larger ones (redone), which may go past the budget for that re-run.  The
+ * resident automata image is not part of the budget. */
+typedef struct {
+  uint64_t budget_bytes;
+  uint64_t queued, inflight_requests;
+  uint64_t inflight_bytes, held_bytes;
+  uint64_t peak_inflight_bytes, peak_held_bytes;
+  uint64_t launches;      /* launches started */
+  uint64_t rejected;      /* requests failed: larger than the budget alone */
+  uint64_t redone;        /* launches re-run after an overflow */
+} ngap_server_info;
+int ngap_server_stats(ngap_server *srv, ngap_server_info *info);
 
 /* ---- learned schedule: frontier-size predictor (no reference counterpart) --
  * A 264 -> 64 -> 64 -> 1 ReLU MLP in FP32 predicting log(1 + items) of a

@@ -27,6 +27,8 @@ ALGORITHMS = {  # --algorithm names (code/src/ngap/main.cu:114-130)
 }
 SCHED_PLAIN = 0
 SCHED_BATCHED = 1
+BUILD_DEVICE = 0   # ngap_load_groups_ex: image built by HIP kernels
+BUILD_HOST = 1     # ... by host threads
 
 
 class GroupDesc(C.Structure):
@@ -91,6 +93,32 @@ class RunStats(C.Structure):
     ]
 
 
+class ImageInfo(C.Structure):
+    _fields_ = [
+        ("built_on_device", C.c_int32),
+        ("memo_depth", C.c_int32),
+        ("num_groups", C.c_int32),
+        ("num_states", C.c_uint64),
+        ("bytes", C.c_uint64),
+        ("build_ms", C.c_double),
+    ]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_uint64),
+        ("queued", C.c_uint64),
+        ("inflight_requests", C.c_uint64),
+        ("inflight_bytes", C.c_uint64),
+        ("held_bytes", C.c_uint64),
+        ("peak_inflight_bytes", C.c_uint64),
+        ("peak_held_bytes", C.c_uint64),
+        ("launches", C.c_uint64),
+        ("rejected", C.c_uint64),
+        ("redone", C.c_uint64),
+    ]
+
+
 # Every symbol include/ngap_amd.h declares, with (restype, argtypes).
 SIGNATURES = {
     "ngap_last_error": (C.c_char_p, []),
@@ -98,6 +126,11 @@ SIGNATURES = {
     "ngap_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     "ngap_destroy": (C.c_int, [C.c_void_p]),
     "ngap_load_groups": (C.c_int, [C.c_void_p, C.POINTER(GroupDesc), C.c_int32, C.c_int32]),
+    "ngap_load_groups_ex": (C.c_int, [C.c_void_p, C.POINTER(GroupDesc), C.c_int32, C.c_int32,
+                                      C.c_int32]),
+    "ngap_image_info": (C.c_int, [C.c_void_p, C.POINTER(ImageInfo)]),
+    "ngap_image_array": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64,
+                                   C.POINTER(C.c_uint64)]),
     "ngap_set_input": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32]),
     "ngap_run": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     "ngap_autotune": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.c_int32, C.POINTER(RunOpts),
@@ -116,6 +149,7 @@ SIGNATURES = {
     "ngap_server_take": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32), C.c_void_p,
                                    C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "ngap_server_pending": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "ngap_server_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerInfo)]),
     "ngap_mlp_create": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "ngap_mlp_destroy": (C.c_int, [C.c_void_p]),
     "ngap_mlp_num_params": (C.c_int32, []),

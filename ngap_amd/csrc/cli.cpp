@@ -76,6 +76,7 @@ struct Options {
   std::string schedule = "batched";
   int window = 0;
   bool write_reports = false;
+  unsigned spill_capacity = 0;
 };
 
 bool parse_bool(const std::string &v, bool &out) {
@@ -143,6 +144,7 @@ bool parse_args(int argc, char **argv, Options &o, std::string &err) {
   opt["--schedule"] = S(o.schedule);
   opt["--window"] = I(o.window);
   opt["--write-reports"] = B(o.write_reports);
+  opt["--spill-capacity"] = [&](const std::string &v) { return parse_num(v, o.spill_capacity); };
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     if (a == "-h" || a == "-?" || a == "--help") {
@@ -186,7 +188,8 @@ void print_help() {
                "  algorithms: blockinggroups NAPgroups nonblockinggroups nonblockingr1groups\n"
                "              nonblockingr2groups nonblockingpcgroups nonblockingallgroups\n"
                "  options of the reference ngap binary are accepted with the same names and\n"
-               "  defaults; added: --schedule=plain|batched --window=N --write-reports=true\n";
+               "  defaults; added: --schedule=plain|batched --window=N --write-reports=true\n"
+               "  --spill-capacity=N (worklist items per wave beyond its LDS ring)\n";
 }
 
 bool read_file(const std::string &fn, std::vector<uint8_t> &out) {
@@ -371,6 +374,15 @@ int main(int argc, char **argv) {
   ro.add_aas_interval = o.add_aas_interval;
   ro.result_capacity = o.output_capacity;
   ro.window = o.window;
+  ro.spill_capacity = o.spill_capacity;
+  // reference scheduling knobs this engine has no counterpart for: the
+  // all-input roots of a step cover a wave's width of positions (one for
+  // NAPgroups) and the worklist is the wave's LDS ring plus its spill area
+  if (o.add_aas_start != 0 || o.add_aas_interval != 1 || o.data_buffer_fetch_size != 128 ||
+      o.adaptive_aas)
+    fprintf(stderr,
+            "note: --add-aan-start, --add-aas-interval, --data-buffer-fetch-size and "
+            "--adaptive-aas are accepted and have no effect on this engine\n");
 
   std::vector<uint64_t> ref;
   bool ref_done = false;
@@ -419,13 +431,22 @@ int main(int argc, char **argv) {
           printf("\n############ Validate result ############ \n");
           pass = validate(res, ref);
         }
-        if (o.write_reports) {
-          FILE *f = fopen(o.report_filename.c_str(), "w");
-          if (f) {
-            for (uint64_t r : res) fprintf(f, "0x%lx\n", (unsigned long)r);
-            fclose(f);
-          }
-        }
+      }
+    }
+    if (rc == NGAP_OK && o.write_reports && !o.report_off) {
+      // sorted unique reports, one per line as the first 40 are printed
+      uint64_t n = 0;
+      ngap_get_reports(eng, nullptr, 0, &n);
+      std::vector<uint64_t> res(n);
+      ngap_get_reports(eng, res.data(), n, &n);
+      FILE *f = fopen(o.report_filename.c_str(), "w");
+      if (!f) {
+        std::cout << BOLDRED << "cannot write " << o.report_filename << RESET << "\n";
+        pass = false;
+      } else {
+        for (uint64_t r : res) fprintf(f, "0x%lx\n", (unsigned long)r);
+        fclose(f);
+        std::cout << "Reports written: " << res.size() << " to " << o.report_filename << "\n";
       }
     }
     if (pass)
@@ -480,11 +501,14 @@ int main(int argc, char **argv) {
     bool pass = false;
     one_run(ro, pass);
     if (!pass && o.try_adaptive_aas) {
-      // retry with a larger worklist spill area (the reference retries with
-      // adaptive AAS batches, ngap.cu:2057-2066)
+      // retry with the default worklist spill area and the engine's overflow
+      // fallback (work items that overflow again re-run with 16x larger
+      // worklists); the reference retries with adaptive AAS batches
+      // (ngap.cu:2057-2066)
       printf("Try adaptive aas\n");
       ngap_run_opts t = ro;
-      t.spill_capacity = 1u << 20;
+      t.spill_capacity = 0;
+      t.fallback = 1;
       one_run(t, pass);
     }
   }

@@ -55,15 +55,22 @@ struct KParams {
   const uint8_t *input;
   uint32_t chunk_len, num_seg, num_groups;
   uint64_t chunk_magic;    // ceil(2^64 / chunk_len): divisibility test (chunk_len > 0)
+  // Request runs (chunk_len 0): bit p set when position p is the last of its
+  // request, so an item of any request finds its chunk end (nullable: then a
+  // wave's current work item gives it)
+  const uint32_t *last_map;
   int32_t memo_depth;      // 1..3, d-gram roots
   int32_t is_dup;
   int32_t dedup;
   int32_t priv_limit;      // in-lane continuation steps, -1 = unlimited
+  int32_t active_threshold;  // continue in-lane only while more lanes than
+                             // this are busy (--active-threshold)
   int32_t report_off;
   uint32_t ring_cap;       // LDS ring entries per wave (power of two)
   uint32_t rbuf_cap;       // LDS report buffer entries per wave
   uint32_t dc_cap;         // LDS dedup cache entries per wave (power of two)
-  uint32_t root_batch;     // plain: positions claimed per cursor step
+  uint32_t root_batch;     // positions whose roots one step takes (per-group
+                           // roots; 1 = NAPgroups' one position at a time)
   unsigned long long *report_count;
   uint64_t *reports;
   uint64_t report_cap;
@@ -96,6 +103,14 @@ constexpr uint32_t kMaxOpen = 8;  // work items a wave overlaps (k_batched)
 // 128-byte line.  A wave whose queue is empty takes from the next ones.
 constexpr uint32_t kQueues = 8, kQueueStride = 32;
 constexpr uint32_t kBlockThreads = 256;
+
+// Batched kernel a launch of P runs: 0 k_batched (one open work item per
+// wave: per-work-item traces, or request runs without a last-byte map), 1
+// k_batched_stream, 2 k_expand (after the root pass).
+NGAP_HD inline int batched_kind(const KParams &P) {
+  if (P.task_items || (!P.chunk_len && !P.last_map)) return 0;
+  return P.root_items ? 2 : 1;
+}
 
 // LDS bytes per wave for the given capacities
 NGAP_HD inline uint32_t wave_lds_bytes(uint32_t ring, uint32_t rbuf, uint32_t dc) {

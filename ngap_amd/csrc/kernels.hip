@@ -625,16 +625,24 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
   const uint32_t b = P.input[valid ? p1 : 0u];
   Rec rec = load_rec(P, valid ? lg.rec_base + v : 0u);
   if (!valid) rec = empty_rec();
-  W.items += __popcll(wballot(valid));
+  const uint32_t busy = __popcll(wballot(valid));
+  W.items += busy;
   // is p1 the last position of its chunk (no successors enabled after it)?
   // Engine runs have chunks of chunk_len bytes: a divisibility test, valid
-  // for items of any work item; request runs use the work item's bounds.
-  const bool last = P.chunk_len ? (uint64_t)(p1 + 1) * P.chunk_magic <= P.chunk_magic - 1
-                                : p1 + 1 >= W.cend;
+  // for items of any work item; request runs read the last-byte map (any
+  // request) or use the wave's one open work item's bounds.
+  const uint32_t lw = P.last_map ? P.last_map[valid ? p1 >> 5 : 0u] : 0u;
+  const bool last = P.chunk_len  ? (uint64_t)(p1 + 1) * P.chunk_magic <= P.chunk_magic - 1
+                    : P.last_map ? ((lw >> (p1 & 31)) & 1u) != 0u
+                                 : p1 + 1 >= W.cend;
   bool has_next = false;
   uint32_t nv = 0;
   bool more = valid;
-  const bool can_priv = P.priv_limit < 0 || depth < P.priv_limit;
+  // work privatization (kernel_ngap_O4.cu:115-122): a matched successor is
+  // expanded by the same lane while more than active_threshold lanes are
+  // busy, and otherwise queued for the idle lanes
+  const bool can_priv = (P.priv_limit < 0 || depth < P.priv_limit) &&
+                        (int32_t)busy > P.active_threshold;
   while (wballot(more)) {
     int32_t link = -1;
     const int32_t es[4] = {rec.ids.x, rec.ids.y, rec.ids.z, rec.ids.w};
@@ -868,8 +876,9 @@ struct TaskStream {
       end = T.end;
     }
     a = cur;
-    // roots_cross takes two halves; the root pass's items go 64 at a time
-    const uint32_t step = cross && !RootItems ? 2 * kWave : kWave;
+    // roots_cross takes two halves; the root pass's items go 64 at a time,
+    // per-group roots root_batch positions at a time
+    const uint32_t step = RootItems ? kWave : cross ? 2 * kWave : P.root_batch;
     b = cur + step < end ? cur + step : end;
     cur = b;
     return 0;
@@ -1286,9 +1295,10 @@ hipError_t launch_roots(hipStream_t s, const KParams &P, uint32_t lds) {
 }
 
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds) {
-  if (P.chunk_len && !P.task_items && P.root_items)
+  const int kind = batched_kind(P);
+  if (kind == 2)
     hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(kBlockThreads), lds, s, P);
-  else if (P.chunk_len && !P.task_items)
+  else if (kind == 1)
     hipLaunchKernelGGL(k_batched_stream, dim3(blocks), dim3(kBlockThreads), lds, s, P);
   else
     hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);

@@ -9,14 +9,29 @@
 
 namespace ngapamd {
 
-uint64_t Image::bytes() const {
-  uint64_t b = meta.size() * sizeof(GroupMeta) + succ4.size() * 4 + cls.size() * 4 +
-               sod.size() * 4;
-  for (auto &m : memo)
-    b += (m.beg.size() + m.nrep.size() + m.sel.size() + m.list.size()) * 4;
-  b += (mask1.size() + xr1.size() + xe1.size() + xmask1.size() + xr2.size() + xe2.size() +
-        xsod.size() + xs3.size() + xp3.size() + x2rep.size()) * 4;
-  return b;
+ImageSizes sizes_of(const Image &img) {
+  ImageSizes z;
+  z.meta = img.meta.size() * sizeof(GroupMeta);
+  z.succ4 = img.succ4.size() * 4;
+  z.cls = img.cls.size() * 4;
+  z.sod = img.sod.size() * 4;
+  for (int d = 0; d < kMaxMemoDepth; d++) {
+    z.beg[d] = img.memo[d].beg.size() * 4;
+    z.nrep[d] = img.memo[d].nrep.size() * 4;
+    z.list[d] = img.memo[d].list.size() * 4;
+    z.sel[d] = img.memo[d].sel.size() * 4;
+  }
+  z.mask1 = img.mask1.size() * 4;
+  z.xr1 = img.xr1.size() * 4;
+  z.xe1 = img.xe1.size() * 4;
+  z.xmask1 = img.xmask1.size() * 4;
+  z.xr2 = img.xr2.size() * 4;
+  z.xe2 = img.xe2.size() * 4;
+  z.xsod = img.xsod.size() * 4;
+  z.xs3 = img.xs3.size() * 4;
+  z.xp3 = img.xp3.size() * 4;
+  z.x2rep = img.x2rep.size() * 4;
+  return z;
 }
 
 namespace {
@@ -61,7 +76,6 @@ struct GroupPart {
   uint32_t mask1[16] = {0};
   MemoLevel memo[kMaxMemoDepth];     // beg from 0, sel ranks local
   std::vector<uint32_t> key2, key3;  // keys of the depth-2 / depth-3 lists
-  uint32_t edges = 0, wide = 0, selfloop = 0, report = 0;
 };
 
 bool fail_part(GroupPart &P, const std::string &m) {
@@ -98,18 +112,12 @@ bool build_part(const HostGroup &hg, int32_t g, int memo_depth, GroupPart &P) {
   for (int v = 0; v < n; v++) {
     for (int e = hg.row[v]; e < hg.row[v + 1]; e++) {
       int u = hg.col[e];
-      if (u == v) P.selfloop++;
       if (is_aas[u]) continue;
       if (std::find(succ[v].begin(), succ[v].end(), u) == succ[v].end()) succ[v].push_back(u);
     }
     std::sort(succ[v].begin(), succ[v].end());
     for (int u : succ[v]) indeg[u]++;
-    int cnt = 0;
-    for (int w = 0; w < 8; w++) cnt += __builtin_popcount(hg.sets[(size_t)v * 8 + w]);
-    if (cnt >= 128) P.wide++;
-    if (hg.attrs[v] & 2) P.report++;
   }
-  P.edges = (uint32_t)E;
   std::vector<int32_t> tag(n);
   for (int v = 0; v < n; v++) {
     int32_t t = v;
@@ -293,12 +301,6 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
       grp3.push_back((uint32_t)g);
     }
     img.meta.push_back(gm);
-    img.stat_edges.push_back(P.edges);
-    img.stat_aas.push_back(hg.num_aas);
-    img.stat_sod.push_back(hg.num_sod);
-    img.stat_report.push_back(P.report);
-    img.stat_wide.push_back(P.wide);
-    img.stat_selfloop.push_back(P.selfloop);
     img.total_states += n;
     P = GroupPart();  // release as we go
   }
@@ -387,6 +389,7 @@ bool build_image(const HostGroup *groups, int32_t num_groups, int32_t memo_depth
         img.xs3[2 * (size_t)k + 1] |= kSlotPrefixReports;
     }
   }
+  img.n_xsod = img.xsod.size() / 2;
   for (auto &m : img.memo)
     if (m.list.size() >= (1ull << 31)) {
       err = "memo tables too large (2^31 entries); use a smaller precompute depth";

@@ -123,12 +123,26 @@ struct Image {
   //       x2rep marks the 2-grams that have a reporting depth-2 list (2048
   //       words).
   std::vector<uint32_t> xr1, xe1, xmask1, xr2, xe2, xsod, xs3, xp3, x2rep;
-  // statistics (for the learned scheduler's features and reports)
-  std::vector<uint32_t> stat_edges, stat_aas, stat_sod, stat_report,
-      stat_wide, stat_selfloop;
+  uint64_t n_xsod = 0;          // (state, group) pairs in xsod
   uint64_t total_states = 0;
-  uint64_t bytes() const;
 };
+
+// Bytes of each image array as built (the device builder keeps the arrays on
+// the device only; the host builder's vectors give the same numbers).
+struct ImageSizes {
+  uint64_t meta = 0, succ4 = 0, cls = 0, sod = 0;
+  uint64_t beg[kMaxMemoDepth] = {0, 0, 0}, nrep[kMaxMemoDepth] = {0, 0, 0},
+           list[kMaxMemoDepth] = {0, 0, 0}, sel[kMaxMemoDepth] = {0, 0, 0};
+  uint64_t mask1 = 0, xr1 = 0, xe1 = 0, xmask1 = 0, xr2 = 0, xe2 = 0, xsod = 0, xs3 = 0,
+           xp3 = 0, x2rep = 0;
+  uint64_t total() const {
+    uint64_t t = meta + succ4 + cls + sod + mask1 + xr1 + xe1 + xmask1 + xr2 + xe2 + xsod + xs3 +
+                 xp3 + x2rep;
+    for (int d = 0; d < kMaxMemoDepth; d++) t += beg[d] + nrep[d] + list[d] + sel[d];
+    return t;
+  }
+};
+ImageSizes sizes_of(const Image &img);
 
 // Validates the groups (ids < 2^22, <= 1024 groups, CSR bounds) and builds the
 // image.  Returns false with err on bad input.

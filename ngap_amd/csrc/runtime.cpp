@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <memory>
 
@@ -96,25 +97,32 @@ bool map_algorithm(int algorithm, int depth_req, int loaded_depth, Mapping &m,
                    std::string &err) {
   // --algorithm -> engine features: each name keeps the optimisations the
   // reference adds at that step (main.cu:114-130, README.md:141-150).
+  // NAPgroups injects the all-input roots of one position at a time
+  // (kernel_bap.cu:98-105, a fake item of capacity 1); the later algorithms
+  // take a batch of positions per step (kernel_ngap_O1.cu:84-112), a wave's
+  // width here.
+  const int batch = (int)kWave;
   switch (algorithm) {
     case NGAP_ALGO_NAP:
+      m = {1, 0, 1};
+      break;
     case NGAP_ALGO_NONBLOCKING:
-      m = {1, 0};
+      m = {1, 0, batch};
       break;
     case NGAP_ALGO_R1:
-      m = {1, 1};
+      m = {1, 1, batch};
       break;
     case NGAP_ALGO_R2:
-      m = {1, 2};
+      m = {1, 2, batch};
       break;
     case NGAP_ALGO_PC:
-      m = {depth_req < 1 ? 1 : depth_req, 0};
+      m = {depth_req < 1 ? 1 : depth_req, 0, batch};
       break;
     case NGAP_ALGO_ALL:
-      m = {depth_req < 1 ? 1 : depth_req, -1};
+      m = {depth_req < 1 ? 1 : depth_req, -1, batch};
       break;
     case NGAP_ALGO_BLOCKING:
-      m = {1, 0};
+      m = {1, 0, batch};
       break;
     default:
       err = "unknown algorithm";
@@ -154,17 +162,18 @@ KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_op
   P.img.xs3 = eng->d_xs3.as<uint32_t>();
   P.img.xp3 = eng->d_xp3.as<uint32_t>();
   P.img.x2rep = eng->d_x2rep.as<uint32_t>();
-  P.img.n_xsod = (uint32_t)(eng->img.xsod.size() / 2);
+  P.img.n_xsod = (uint32_t)eng->img.n_xsod;
   P.num_groups = (uint32_t)eng->img.num_groups;
   P.memo_depth = mp.memo_depth;
   P.is_dup = o.is_dup;
   P.dedup = o.unique;
   P.priv_limit = mp.priv_limit;
+  P.active_threshold = o.active_threshold;
   P.report_off = o.report_off;
   P.ring_cap = o.schedule == NGAP_SCHED_BATCHED && ring_of(o) ? ring_of(o) : kRing;
   P.rbuf_cap = kRbuf;
   P.dc_cap = kDc;
-  P.root_batch = kWave;
+  P.root_batch = (uint32_t)mp.root_batch;
   P.waves_per_block = kBlockThreads / kWave;
   return P;
 }
@@ -211,7 +220,7 @@ static int root_areas(const ngap_engine *eng, RunCtx &ctx, const std::vector<Tas
     return NGAP_OK;
   }
   ctx.root_factor = 0;
-  const uint64_t n_xsod = eng->img.xsod.size() / 2;
+  const uint64_t n_xsod = eng->img.n_xsod;
   std::vector<uint64_t> off(tasks.size() + 1, 0);
   std::vector<RootPiece> pieces;
   for (size_t t = 0; t < tasks.size(); t++) {
@@ -329,8 +338,7 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   uint32_t blocks = 0;
   // engine runs without traces launch the streaming kernel (k_expand after a
   // root pass), the others the per-work-item kernel (launch_batched)
-  rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap,
-                         P.root_items ? 2 : (P.chunk_len && !P.task_items ? 1 : 0));
+  rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap, batched_kind(P));
   if (rc) return rc;
   P.spill_cap = o.spill_capacity ? o.spill_capacity : 65536;
   HIP_TRY(ctx.spill.reserve((size_t)blocks * P.waves_per_block * P.spill_cap * 8));
@@ -667,10 +675,13 @@ int ngap_destroy(ngap_engine *eng) {
   return NGAP_OK;
 }
 
-int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t num_groups,
-                     int32_t max_depth) {
+int ngap_load_groups_ex(ngap_engine *eng, const ngap_group_desc *groups, int32_t num_groups,
+                        int32_t max_depth, int32_t build) {
   if (!eng || !groups) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  if (build != NGAP_BUILD_DEVICE && build != NGAP_BUILD_HOST)
+    return ngap_fail(NGAP_ERR_INVALID, "build must be NGAP_BUILD_DEVICE or NGAP_BUILD_HOST");
   HIP_TRY(hipSetDevice(eng->device));
+  const auto t0 = std::chrono::steady_clock::now();
   std::vector<HostGroup> hg(num_groups > 0 ? num_groups : 0);
   for (int i = 0; i < num_groups; i++) {
     hg[i].num_states = groups[i].num_states;
@@ -685,34 +696,109 @@ int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t nu
   }
   std::string err;
   eng->loaded = false;
-  if (!build_image(hg.data(), num_groups, max_depth < 1 ? 1 : max_depth, eng->img, err))
-    return ngap_fail(NGAP_ERR_INVALID, err);
-  const Image &im = eng->img;
-  HIP_TRY(eng->d_meta.upload(im.meta));
-  HIP_TRY(eng->d_succ4.upload(im.succ4));
-  HIP_TRY(eng->d_cls.upload(im.cls));
-  HIP_TRY(eng->d_sod.upload(im.sod));
-  HIP_TRY(eng->d_mask1.upload(im.mask1));
-  HIP_TRY(eng->d_xr1.upload(im.xr1));
-  HIP_TRY(eng->d_xe1.upload(im.xe1));
-  HIP_TRY(eng->d_xmask1.upload(im.xmask1));
-  HIP_TRY(eng->d_xr2.upload(im.xr2));
-  HIP_TRY(eng->d_xe2.upload(im.xe2));
-  HIP_TRY(eng->d_xsod.upload(im.xsod));
-  HIP_TRY(eng->d_xs3.upload(im.xs3));
-  HIP_TRY(eng->d_xp3.upload(im.xp3));
-  HIP_TRY(eng->d_x2rep.upload(im.x2rep));
-  for (int d = 0; d < kMaxMemoDepth; d++) {
-    HIP_TRY(eng->d_beg[d].upload(im.memo[d].beg));
-    HIP_TRY(eng->d_nrep[d].upload(im.memo[d].nrep));
-    HIP_TRY(eng->d_list[d].upload(im.memo[d].list));
-    HIP_TRY(eng->d_sel[d].upload(im.memo[d].sel));
-  }
-  eng->loaded = true;
   eng->generation++;
   eng->ctx.have_run = false;
   eng->ctx.drop_graph();
+  const int depth = max_depth < 1 ? 1 : max_depth;
+  if (build == NGAP_BUILD_DEVICE) {
+    const int rc = build_image_device(eng, hg.data(), num_groups, depth, eng->ctx.stream,
+                                      eng->img, eng->img_sizes, err);
+    if (rc) return err.empty() ? rc : ngap_fail(rc, err);
+  } else {
+    if (!build_image(hg.data(), num_groups, depth, eng->img, err))
+      return ngap_fail(NGAP_ERR_INVALID, err);
+    Image &im = eng->img;
+    HIP_TRY(eng->d_meta.upload(im.meta));
+    HIP_TRY(eng->d_succ4.upload(im.succ4));
+    HIP_TRY(eng->d_cls.upload(im.cls));
+    HIP_TRY(eng->d_sod.upload(im.sod));
+    HIP_TRY(eng->d_mask1.upload(im.mask1));
+    HIP_TRY(eng->d_xr1.upload(im.xr1));
+    HIP_TRY(eng->d_xe1.upload(im.xe1));
+    HIP_TRY(eng->d_xmask1.upload(im.xmask1));
+    HIP_TRY(eng->d_xr2.upload(im.xr2));
+    HIP_TRY(eng->d_xe2.upload(im.xe2));
+    HIP_TRY(eng->d_xsod.upload(im.xsod));
+    HIP_TRY(eng->d_xs3.upload(im.xs3));
+    HIP_TRY(eng->d_xp3.upload(im.xp3));
+    HIP_TRY(eng->d_x2rep.upload(im.x2rep));
+    for (int d = 0; d < kMaxMemoDepth; d++) {
+      HIP_TRY(eng->d_beg[d].upload(im.memo[d].beg));
+      HIP_TRY(eng->d_nrep[d].upload(im.memo[d].nrep));
+      HIP_TRY(eng->d_list[d].upload(im.memo[d].list));
+      HIP_TRY(eng->d_sel[d].upload(im.memo[d].sel));
+    }
+    eng->img_sizes = sizes_of(im);
+    // the host keeps what the runtime reads (group metadata, root masks)
+    Image kept;
+    kept.num_groups = im.num_groups;
+    kept.memo_depth = im.memo_depth;
+    kept.meta.swap(im.meta);
+    kept.mask1.swap(im.mask1);
+    kept.n_xsod = im.n_xsod;
+    kept.total_states = im.total_states;
+    im = std::move(kept);
+  }
+  eng->built_on_device = build == NGAP_BUILD_DEVICE;
+  eng->build_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  eng->loaded = true;
+  eng->generation++;
   return NGAP_OK;
+}
+
+int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t num_groups,
+                     int32_t max_depth) {
+  return ngap_load_groups_ex(eng, groups, num_groups, max_depth, NGAP_BUILD_HOST);
+}
+
+int ngap_image_info(ngap_engine *eng, ngap_image_summary *info) {
+  if (!eng || !info) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
+  memset(info, 0, sizeof *info);
+  info->built_on_device = eng->built_on_device ? 1 : 0;
+  info->memo_depth = eng->img.memo_depth;
+  info->num_groups = eng->img.num_groups;
+  info->num_states = eng->img.total_states;
+  info->bytes = eng->img_sizes.total();
+  info->build_ms = eng->build_ms;
+  return NGAP_OK;
+}
+
+int ngap_image_array(ngap_engine *eng, const char *name, void *out, uint64_t cap,
+                     uint64_t *n_bytes) {
+  if (!eng || !name || !n_bytes) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+  if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
+  const ImageSizes &z = eng->img_sizes;
+  struct Entry {
+    const char *name;
+    const DevBuf *buf;
+    uint64_t bytes;
+  };
+  const Entry table[] = {
+      {"meta", &eng->d_meta, z.meta},       {"succ4", &eng->d_succ4, z.succ4},
+      {"cls", &eng->d_cls, z.cls},          {"sod", &eng->d_sod, z.sod},
+      {"beg1", &eng->d_beg[0], z.beg[0]},   {"beg2", &eng->d_beg[1], z.beg[1]},
+      {"beg3", &eng->d_beg[2], z.beg[2]},   {"nrep1", &eng->d_nrep[0], z.nrep[0]},
+      {"nrep2", &eng->d_nrep[1], z.nrep[1]}, {"nrep3", &eng->d_nrep[2], z.nrep[2]},
+      {"list1", &eng->d_list[0], z.list[0]}, {"list2", &eng->d_list[1], z.list[1]},
+      {"list3", &eng->d_list[2], z.list[2]}, {"sel2", &eng->d_sel[1], z.sel[1]},
+      {"sel3", &eng->d_sel[2], z.sel[2]},   {"mask1", &eng->d_mask1, z.mask1},
+      {"xr1", &eng->d_xr1, z.xr1},          {"xe1", &eng->d_xe1, z.xe1},
+      {"xmask1", &eng->d_xmask1, z.xmask1}, {"xr2", &eng->d_xr2, z.xr2},
+      {"xe2", &eng->d_xe2, z.xe2},          {"xsod", &eng->d_xsod, z.xsod},
+      {"xs3", &eng->d_xs3, z.xs3},          {"xp3", &eng->d_xp3, z.xp3},
+      {"x2rep", &eng->d_x2rep, z.x2rep}};
+  for (const Entry &e : table) {
+    if (strcmp(e.name, name) != 0) continue;
+    *n_bytes = e.bytes;
+    if (out && e.bytes) {
+      HIP_TRY(hipSetDevice(eng->device));
+      HIP_TRY(hipMemcpy(out, e.buf->p, std::min<uint64_t>(cap, e.bytes), hipMemcpyDeviceToHost));
+    }
+    return NGAP_OK;
+  }
+  return ngap_fail(NGAP_ERR_INVALID, std::string("unknown image array ") + name);
 }
 
 int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
@@ -727,15 +813,27 @@ int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
   size_t n = (size_t)chunk_len * num_seg;
   HIP_TRY(eng->d_input.reserve(n + 16));
   HIP_TRY(hipMemcpy(eng->d_input.p, data, n, hipMemcpyHostToDevice));
-  uint64_t hist[256] = {0};
-  for (size_t i = 0; i < n; i++) hist[data[i]]++;
-  for (int b = 0; b < 256; b++) eng->byte_freq[b] = (double)hist[b] / (double)n;
+  // byte histogram for the static per-group schedule (four partial counts:
+  // no store-to-load dependence between neighbouring bytes)
+  uint32_t hist[4][256] = {{0}};
+  size_t i = 0;
+  for (; i + 4 <= n; i += 4) {
+    hist[0][data[i]]++;
+    hist[1][data[i + 1]]++;
+    hist[2][data[i + 2]]++;
+    hist[3][data[i + 3]]++;
+  }
+  for (; i < n; i++) hist[0][data[i]]++;
+  for (int b = 0; b < 256; b++)
+    eng->byte_freq[b] =
+        ((double)hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b]) / (double)n;
   eng->chunk_len = chunk_len;
   eng->num_seg = num_seg;
   eng->have_input = true;
   eng->generation++;
   eng->ctx.have_run = false;
-  eng->ctx.drop_graph();
+  // a captured graph stays: its launch parameters (input address, chunk
+  // sizes) are compared with the next run's before it is replayed
   return NGAP_OK;
 }
 

@@ -85,6 +85,7 @@ struct RunCtx {
 struct Mapping {
   int memo_depth;
   int priv_limit;
+  int root_batch;  // positions per root step of per-group work
 };
 
 bool map_algorithm(int algorithm, int depth_req, int loaded_depth, Mapping &m, std::string &err);
@@ -132,6 +133,9 @@ struct ngap_engine {
     std::vector<ngapamd::Task> tasks;
   } sched;
   uint64_t generation = 1;      // bumped by ngap_load_groups / ngap_set_input
+  ngapamd::ImageSizes img_sizes;  // bytes of each device image array
+  bool built_on_device = false;   // the last ngap_load_groups built on the device
+  double build_ms = 0;            // its wall time
 };
 
 namespace ngapamd {
@@ -148,4 +152,11 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
               const ngap_run_opts &o, ngap_run_stats *st);
 // Device reports of ctx's last run, sorted and unique (device encoding).
 int sorted_reports(RunCtx &ctx, std::vector<uint64_t> &out);
+// Builds the image of the groups on the device into eng's image buffers
+// (build.hip): the arrays build_image makes, byte for byte.  img gets the
+// parts the host reads (group metadata, root masks, counts), sz the arrays'
+// sizes.  Returns a status; err holds the message.
+int build_image_device(ngap_engine *eng, const HostGroup *groups, int32_t num_groups,
+                       int32_t depth, hipStream_t s, Image &img, ImageSizes &sz,
+                       std::string &err);
 }  // namespace ngapamd

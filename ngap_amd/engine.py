@@ -35,12 +35,39 @@ class Engine:
     def __exit__(self, *a):
         self.close()
 
-    def load_groups(self, groups, max_precompute_depth=2):
+    def load_groups(self, groups, max_precompute_depth=2, build=None):
+        """Builds the automata image: ``build`` "device" (HIP kernels),
+        "host" (host threads) or None (ngap_load_groups' choice); both give
+        the same arrays."""
         descs = (_lib.GroupDesc * len(groups))(*[g.desc() for g in groups])
-        check(_lib.lib().ngap_load_groups(self._h, descs, len(groups),
-                                          int(max_precompute_depth)))
+        if build is None:
+            check(_lib.lib().ngap_load_groups(self._h, descs, len(groups),
+                                              int(max_precompute_depth)))
+        else:
+            where = {"device": _lib.BUILD_DEVICE, "host": _lib.BUILD_HOST}[build]
+            check(_lib.lib().ngap_load_groups_ex(self._h, descs, len(groups),
+                                                 int(max_precompute_depth), where))
         self.num_groups = len(groups)
         self.max_depth = max(1, int(max_precompute_depth))
+
+    IMAGE_ARRAYS = ("meta", "succ4", "cls", "sod", "beg1", "beg2", "beg3", "nrep1", "nrep2",
+                    "nrep3", "list1", "list2", "list3", "sel2", "sel3", "mask1", "xr1", "xe1",
+                    "xmask1", "xr2", "xe2", "xsod", "xs3", "xp3", "x2rep")
+
+    def image_info(self):
+        info = _lib.ImageInfo()
+        check(_lib.lib().ngap_image_info(self._h, C.byref(info)))
+        return {f: getattr(info, f) for f, _ in info._fields_}
+
+    def image_array(self, name):
+        """A device image array as raw bytes (numpy uint8)."""
+        n = C.c_uint64()
+        check(_lib.lib().ngap_image_array(self._h, name.encode(), None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint8)
+        if n.value:
+            check(_lib.lib().ngap_image_array(self._h, name.encode(), out.ctypes.data, n.value,
+                                              C.byref(n)))
+        return out
 
     def set_input(self, data, chunk_len, num_seg=None):
         buf = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8)

@@ -63,6 +63,12 @@ class Server:
         check(_lib.lib().ngap_server_pending(self._h, C.byref(q), C.byref(f)))
         return q.value, f.value
 
+    def stats(self):
+        """Memory and admission figures (ngap_server_stats) as a dict."""
+        info = _lib.ServerInfo()
+        check(_lib.lib().ngap_server_stats(self._h, C.byref(info)))
+        return {f: getattr(info, f) for f, _ in info._fields_}
+
     def drain(self, ids):
         """Polls until every id is done; returns {id: (reports, latency_ms)}."""
         out = {}

@@ -264,13 +264,20 @@ def drifting_input(rng, input_len, chunk, planted_rate=1e-4, groups=None, period
         parts.append(zipf_bytes(rng, chunk, s))
     data = np.concatenate(parts) if parts else np.zeros(0, np.uint8)
     if groups:
-        specs = [(gi, p) for gi, g in enumerate(groups) for p in getattr(g, "patterns", [])]
-        n_plant = int(planted_rate * input_len)
-        for _ in range(n_plant):
-            gi, spec = specs[int(rng.integers(0, len(specs)))]
-            inst = sample_instance(rng, spec)
-            pos = int(rng.integers(0, max(1, input_len - len(inst))))
-            data[pos:pos + len(inst)] = np.frombuffer(inst, np.uint8)
+        plant(rng, data, groups, planted_rate)
+    return data
+
+
+def plant(rng, data, groups, rate):
+    """Writes instances of the groups' patterns into data (in place) at
+    `rate` per byte, each at a uniform position."""
+    specs = [(gi, p) for gi, g in enumerate(groups) for p in getattr(g, "patterns", [])]
+    n = data.size
+    for _ in range(int(rate * n)):
+        gi, spec = specs[int(rng.integers(0, len(specs)))]
+        inst = sample_instance(rng, spec)
+        pos = int(rng.integers(0, max(1, n - len(inst))))
+        data[pos:pos + len(inst)] = np.frombuffer(inst, np.uint8)[: n - pos]
     return data
 
 
@@ -299,6 +306,54 @@ def serving_requests(seed=2004, n=256, n_groups=64, lo=1 << 10, hi=4 << 20, rate
         L = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         out.append((t, int(rng.integers(0, n_groups)), zipf_bytes(rng, L, 1.1)))
     return out
+
+
+def write_groups_anml(path, groups):
+    """Writes every group's states into one ANML document, group after group
+    in state order, with ids g<group>_<state> and VASim hex-range symbol
+    sets (the front-end reads the same automaton back)."""
+    from .groups import symbol_set_string, _xml_attr
+    cache = {}
+    with open(path, "w") as f:
+        f.write('<anml version="1.0">\n<automata-network id="synthetic">\n')
+        for gi, g in enumerate(groups):
+            sets = np.ascontiguousarray(g.sets).reshape(g.num_states, 8)
+            sod = set(g.sod.tolist())
+            lines = []
+            for v in range(g.num_states):
+                key = sets[v].tobytes()
+                ss = cache.get(key)
+                if ss is None:
+                    bits = np.unpackbits(sets[v].view(np.uint8), bitorder="little").astype(bool)
+                    ss = cache[key] = _xml_attr(symbol_set_string(bits))
+                start = ' start="all-input"' if g.attrs[v] & 1 else \
+                    (' start="start-of-data"' if v in sod else "")
+                lines.append(f'<state-transition-element id="g{gi}_{v}" symbol-set="{ss}"{start}>')
+                for u in g.col[g.row[v]:g.row[v + 1]]:
+                    lines.append(f'<activate-on-match element="g{gi}_{u}"/>')
+                if g.attrs[v] & 2:
+                    lines.append('<report-on-match reportcode="1"/>')
+                lines.append("</state-transition-element>")
+            f.write("\n".join(lines))
+            f.write("\n")
+        f.write("</automata-network>\n</anml>\n")
+
+
+def dump_groups(groups):
+    """The groups in the front-end dump format (ngap_frontend_dump), with the
+    state names write_groups_anml gives them: equal to the front-end's dump of
+    that ANML exactly when loading it gives these groups back unchanged."""
+    out = [f"groups {len(groups)}"]
+    for gi, g in enumerate(groups):
+        n = g.num_states
+        out.append(f"group {gi} states {n} edges {g.num_edges} aas {g.aas.size} "
+                   f"sod {g.sod.size}")
+        sets = np.ascontiguousarray(g.sets).reshape(n, 8)
+        for v in range(n):
+            words = "".join(f"{int(w):08x}" for w in sets[v][::-1])
+            succ = "".join(f" {int(u)}" for u in g.col[g.row[v]:g.row[v + 1]])
+            out.append(f"{v} g{gi}_{v} a{int(g.attrs[v]) & 3} s{words} ->{succ}")
+    return "\n".join(out) + "\n"
 
 
 def to_anml_states(group, prefix="s"):

@@ -43,18 +43,24 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build()
         L = C.CDLL(LIB_PATH)
-        L.oracle_run.restype = C.c_int
-        L.oracle_run.argtypes = [C.c_int32, C.POINTER(OracleGroup), C.c_void_p, C.c_uint32,
-                                 C.c_uint32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
-                                 C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
-                                 C.POINTER(OracleStats)]
+        L.oracle_run_ex.restype = C.c_int
+        L.oracle_run_ex.argtypes = [C.c_int32, C.POINTER(OracleGroup), C.c_void_p, C.c_uint32,
+                                    C.c_uint32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                    C.POINTER(OracleStats)]
         _lib = L
     return _lib
 
 
 def run(groups, data, chunk_len, num_seg=None, is_dup=False, threads=0, unit_mask=None,
-        memo_depth=1):
-    """Reports (uint64 array, reference encoding, sorted unique) and stats."""
+        memo_depth=1, implicit_aas=True):
+    """Reports (uint64 array, reference encoding, sorted unique) and stats.
+
+    implicit_aas=False runs the reference's loop as written (every all-input
+    state in the frontier, tested at every position: the CPU baseline);
+    True leaves them out of the frontier after a chunk's first position and
+    visits those whose class holds the byte, with the same reports and counts
+    (checked by tests/test_oracle.py)."""
     buf = np.frombuffer(bytes(data), np.uint8) if isinstance(data, (bytes, bytearray)) \
         else np.ascontiguousarray(data, np.uint8)
     if num_seg is None:
@@ -73,13 +79,14 @@ def run(groups, data, chunk_len, num_seg=None, is_dup=False, threads=0, unit_mas
     n = C.c_uint64()
     st = OracleStats()
     L = lib()
-    L.oracle_run(len(groups), og, buf.ctypes.data, chunk_len, num_seg, int(is_dup),
-                 mask.ctypes.data if mask is not None else None, int(threads), int(memo_depth),
-                 None, 0, C.byref(n), C.byref(st))
+    L.oracle_run_ex(len(groups), og, buf.ctypes.data, chunk_len, num_seg, int(is_dup),
+                    mask.ctypes.data if mask is not None else None, int(threads),
+                    int(memo_depth), int(bool(implicit_aas)), None, 0, C.byref(n), C.byref(st))
     out = np.zeros(n.value, np.uint64)
-    L.oracle_run(len(groups), og, buf.ctypes.data, chunk_len, num_seg, int(is_dup),
-                 mask.ctypes.data if mask is not None else None, int(threads), int(memo_depth),
-                 out.ctypes.data, n.value, C.byref(n), C.byref(st))
+    L.oracle_run_ex(len(groups), og, buf.ctypes.data, chunk_len, num_seg, int(is_dup),
+                    mask.ctypes.data if mask is not None else None, int(threads),
+                    int(memo_depth), int(bool(implicit_aas)), out.ctypes.data, n.value,
+                    C.byref(n), C.byref(st))
     stats = {k: getattr(st, k) for k, _ in OracleStats._fields_}
     return out, stats
 

@@ -75,11 +75,11 @@ struct oracle_stats {
 // c*num_groups + j) selects units; `threads` <= 0 means all cores.  Reports
 // are written to out (capacity cap) in reference order; *n_out gets the full
 // count even when it exceeds cap.  Returns 0, or 1 when out was too small.
-int oracle_run(int32_t num_groups, const oracle_group *groups,
-               const uint8_t *input, uint32_t chunk_len, uint32_t num_seg,
-               int32_t is_dup, const uint8_t *unit_mask, int32_t threads,
-               int32_t memo_depth, uint64_t *out, uint64_t cap, uint64_t *n_out,
-               oracle_stats *stats_out) {
+int oracle_run_ex(int32_t num_groups, const oracle_group *groups,
+                  const uint8_t *input, uint32_t chunk_len, uint32_t num_seg,
+                  int32_t is_dup, const uint8_t *unit_mask, int32_t threads,
+                  int32_t memo_depth, int32_t implicit_aas, uint64_t *out, uint64_t cap,
+                  uint64_t *n_out, oracle_stats *stats_out) {
   const int K = memo_depth < 1 ? 1 : memo_depth;
   const int64_t units = (int64_t)num_groups * num_seg;
   std::vector<std::vector<uint64_t>> per_unit(units);
@@ -116,6 +116,85 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
     for (int i = 0; i < g.num_aas; i++) cur.emplace_back(g.aas[i], 0);
     merge(cur);
     const uint64_t begin = (uint64_t)c * chunk_len, end = begin + chunk_len;
+    // implicit_aas: the same frontier with the all-input states left out of
+    // it after the first position.  They are enabled at every position with
+    // chain depth 0 (they are re-added with depth 0 and no edge leads to them,
+    // ngap.cu:2233-2256), so each position visits the all-input states whose
+    // class holds its byte (a per-byte list) instead of testing every one, and
+    // counts them as enabled; everything it reports and counts is what the
+    // loop below would.
+    std::vector<std::vector<int32_t>> aas_by_byte;
+    if (implicit_aas && g.num_aas > 0) {
+      aas_by_byte.resize(256);
+      std::vector<int32_t> ids(g.aas, g.aas + g.num_aas);
+      std::sort(ids.begin(), ids.end());
+      ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+      for (int32_t v : ids)
+        for (int b = 0; b < 256; b++)
+          if ((g.sets[(size_t)v * 8 + b / 32] >> (b % 32)) & 1u) aas_by_byte[b].push_back(v);
+    }
+    const int64_t n_aas_distinct = [&] {
+      std::vector<int32_t> ids(g.aas, g.aas + g.num_aas);
+      std::sort(ids.begin(), ids.end());
+      return (int64_t)(std::unique(ids.begin(), ids.end()) - ids.begin());
+    }();
+    if (implicit_aas && g.num_aas > 0 && chunk_len > 0) {
+      // first position: the general loop (start-of-data seeds may share ids
+      // with all-input states); then every later position with the all-input
+      // states implicit
+      std::vector<SD> first;
+      first.swap(cur);
+      for (uint64_t p = begin; p < end; p++) {
+        const uint8_t b = input[p];
+        nxt.clear();
+        const bool at_first = p == begin;
+        const std::vector<SD> &front = at_first ? first : cur;
+        st.enabled_items += front.size() + (at_first ? 0 : (uint64_t)n_aas_distinct);
+        bool root_here = false, list2_here = false, deep_here = false;
+        auto visit = [&](int32_t v, int32_t d) {
+          st.matched_items++;
+          if (is_aas[v]) {
+            st.matched_aas++;
+            root_here = true;
+          }
+          if (g.attrs[v] & 2) {
+            uint32_t id = (uint32_t)v | ((uint32_t)j << 22);
+            rep.push_back(((uint64_t)id << 32) | (uint32_t)p);
+          }
+          const bool materialised = d >= K - 1;
+          if (d == K - 1) st.memo_list_items++;
+          if (d == 1) list2_here = true;
+          if (d == K - 1) deep_here = true;
+          if (p + 1 < end) {
+            st.matched_advance++;
+            st.successors += (uint64_t)(g.row[v + 1] - g.row[v]);
+            if (materialised && nonaas_deg[v] > 0) {
+              st.expanded_items++;
+              st.expanded_tests += nonaas_deg[v];
+            }
+          }
+          const int32_t dn = d + 1 < K ? d + 1 : K;
+          for (int e = g.row[v]; e < g.row[v + 1]; e++)
+            if (!is_aas[g.col[e]]) nxt.emplace_back(g.col[e], dn);
+        };
+        for (const SD &vd : front) {
+          const int32_t v = vd.first, d = vd.second;
+          if (!is_aas[v]) st.tested_non_aas++;
+          if (!((g.sets[(size_t)v * 8 + b / 32] >> (b % 32)) & 1u)) continue;
+          visit(v, d);
+        }
+        if (!at_first)
+          for (int32_t v : aas_by_byte[b]) visit(v, 0);
+        st.root_positions += root_here;
+        st.level2_lists += list2_here;
+        st.deepest_lists += deep_here;
+        merge(nxt);
+        cur.swap(nxt);
+      }
+      st.reports = rep.size();
+      per_stats[u] = st;
+      continue;
+    }
     for (uint64_t p = begin; p < end && !cur.empty(); p++) {
       const uint8_t b = input[p];
       nxt.clear();
@@ -199,6 +278,15 @@ int oracle_run(int32_t num_groups, const oracle_group *groups,
   }
   if (out) std::copy(all.begin(), all.end(), out);
   return 0;
+}
+
+// The reference loop as written (every all-input state in the frontier).
+int oracle_run(int32_t num_groups, const oracle_group *groups, const uint8_t *input,
+               uint32_t chunk_len, uint32_t num_seg, int32_t is_dup, const uint8_t *unit_mask,
+               int32_t threads, int32_t memo_depth, uint64_t *out, uint64_t cap,
+               uint64_t *n_out, oracle_stats *stats_out) {
+  return oracle_run_ex(num_groups, groups, input, chunk_len, num_seg, is_dup, unit_mask,
+                       threads, memo_depth, 0, out, cap, n_out, stats_out);
 }
 
 }  // extern "C"

@@ -55,7 +55,8 @@ def test_ctypes_structs_match_the_header_layout(tmp_path):
     """Size and every field offset of the ctypes mirrors equal what a C
     compiler lays out from include/ngap_amd.h."""
     pairs = [("ngap_task", _lib.Task), ("ngap_group_desc", _lib.GroupDesc),
-             ("ngap_run_opts", _lib.RunOpts), ("ngap_run_stats", _lib.RunStats)]
+             ("ngap_run_opts", _lib.RunOpts), ("ngap_run_stats", _lib.RunStats),
+             ("ngap_server_info", _lib.ServerInfo), ("ngap_image_summary", _lib.ImageInfo)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ngap_amd.h"', "int main(void) {"]
     for cname, py in pairs:
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -272,3 +272,53 @@ def test_server_alone_equals_mix_equals_oracle(engine):
     for i in range(len(reqs)):
         assert np.array_equal(res["reports"][i], expect[i])
     srv.close()
+
+
+def test_server_budget_binds_and_results_stay_exact(engine):
+    """A 64 MiB HBM budget: launches never hold more than the budget, queued
+    requests start as earlier launches free their memory, a request larger
+    than the budget alone fails, and every other result equals the oracle."""
+    groups = [synth.regex_group(np.random.default_rng(60 + i), 1500 + 1000 * i,
+                                dotstar=(i == 1)) for i in range(4)]
+    engine.load_groups(groups, 3)
+    rng = np.random.default_rng(61)
+    reqs = []
+    for i in range(32):
+        L = int(np.exp(rng.uniform(np.log(8 << 10), np.log(1 << 20))))
+        reqs.append((int(rng.integers(0, len(groups))), synth.zipf_bytes(rng, L, 1.1)))
+    expect = []
+    for g, d in reqs:
+        r, _ = oracle.run([groups[g]], d, len(d), 1)
+        expect.append(r | np.uint64(g << 22 << 32))
+    budget = 64 << 20
+    srv = Server(engine, budget_bytes=budget, max_inflight=4, algorithm="nonblockingallgroups",
+                 precompute_depth=3, unique=True)
+    say("server budget: oversized request")
+    big = srv.submit(0, synth.zipf_bytes(rng, 40 << 20, 1.1))
+    say("server budget: 32 requests at once")
+    ids = [srv.submit(g, d) for g, d in reqs]
+    got = srv.drain(ids)
+    st = srv.stats()
+    say("server budget stats", st)
+    for _ in range(100):
+        state, rep, _ = srv.take(big)
+        if state == 3:
+            break
+        srv.poll(block=True)
+    assert state == 3, "a request larger than the budget fails"
+    assert st["rejected"] == 1
+    assert 0 < st["peak_held_bytes"] <= budget
+    assert st["peak_inflight_bytes"] <= budget
+    # about 17 MiB queued at once do not fit one launch under this budget
+    assert st["launches"] >= 3
+    assert st["redone"] == 0
+    for i in range(len(reqs)):
+        assert np.array_equal(got[ids[i]][0], expect[i]), i
+    srv.close()
+
+
+def test_server_rejects_report_off(engine):
+    groups = [synth.regex_group(np.random.default_rng(70), 500)]
+    engine.load_groups(groups, 2)
+    with pytest.raises(NgapError):
+        Server(engine, report_off=True, algorithm="nonblockingallgroups", precompute_depth=2)

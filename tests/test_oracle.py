@@ -148,3 +148,36 @@ def test_memo_depth_restarts_at_all_input_successors():
     assert st["expanded_items"] == 29             # each expands to 'b'
     assert st["reports"] == 29
     assert st["level2_lists"] == 29               # positions of those 'a's
+
+
+def _with_sod_on_all_input(g):
+    """The group with its first all-input state also listed as start-of-data
+    (the frontier then holds it with the start-of-data depth at a chunk's
+    first position)."""
+    from ngap_amd.groups import Group
+    sod = np.unique(np.concatenate([g.sod, g.aas[:1]])).astype(np.int32)
+    return Group(g.row, g.col, g.sets, g.attrs, g.aas, sod)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_implicit_all_input_equals_reference_loop(seed):
+    """The fast mode (all-input states visited through per-byte lists after a
+    chunk's first position) gives the reports and every work count of the
+    reference's loop as written, on automata with dot-star loops, start-of-data
+    states (one of them also all-input), chunks with and without duplication
+    and memo depths 1-3."""
+    from ngap_amd import synth
+    rng = np.random.default_rng(900 + seed)
+    groups = [synth.regex_group(rng, int(s), dotstar=(i % 3 == 0))
+              for i, s in enumerate([120, 900, 2500, 400])]
+    groups.append(_with_sod_on_all_input(H.chains_group(
+        [[(H.LIT, [65]), (H.CLS, [66, 67]), (H.LIT, [68])], [(H.LIT, [66]), (H.LIT, [67])]])[0]))
+    data = synth.drifting_input(rng, 1 << 14, 2048, 5e-3, groups[:4])
+    data[::97] = np.frombuffer(b"ABCD", np.uint8)[np.arange(data[::97].size) % 4]
+    for K in (1, 2, 3):
+        for dup in (False, True):
+            a, sa = oracle.run(groups, data, 2048, is_dup=dup, memo_depth=K, implicit_aas=False)
+            b, sb = oracle.run(groups, data, 2048, is_dup=dup, memo_depth=K, implicit_aas=True)
+            assert a.size > 10
+            assert np.array_equal(a, b), (K, dup)
+            assert sa == sb, (K, dup)
