@@ -480,6 +480,8 @@ def main():
     ap.add_argument("--serving", default="100,1000,10000,inf",
                     help="arrival rates (per second, inf = all at once) for the serving run; "
                          "'none' skips it")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving sweep (prints its JSON object)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle work counts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -493,6 +495,10 @@ def main():
                           "(the (group, chunk) units would shard across GPUs with no exchange)"}))
         sys.exit(2)
 
+    if a.serving_only:
+        print(json.dumps({"serving": serving_bench([float(r) for r in a.serving.split(",")],
+                                                   n_requests=a.requests)}), flush=True)
+        return
     res, runs, groups, data, chunk = compare(a.workload, a.depth, a.steps, a.warmup,
                                              learned_on=not a.no_learned, tune=not a.no_tune,
                                              repeats=a.repeats, check=not a.no_check,

@@ -85,13 +85,18 @@ typedef struct {
   int32_t algorithm;            /* NGAP_ALGO_* */
   int32_t schedule;             /* NGAP_SCHED_* */
   int32_t precompute_depth;     /* --precompute-depth, 0..3 */
-  int32_t active_threshold;     /* --active-threshold */
+  int32_t active_threshold;     /* --active-threshold: a lane continues with a
+                                   matched successor in-lane only while more
+                                   lanes than this are busy (else it queues it) */
   int32_t unique;               /* --unique: de-duplicate (state, offset) items */
   int32_t is_dup;               /* start-of-data states in every chunk */
   int32_t report_off;           /* --report-off: count reports only */
-  uint32_t data_buffer_fetch_size; /* --data-buffer-fetch-size (tuning hint) */
-  uint32_t add_aas_start;       /* --add-aan-start (tuning hint) */
-  uint32_t add_aas_interval;    /* --add-aas-interval: root batch positions */
+  uint32_t data_buffer_fetch_size; /* --data-buffer-fetch-size: accepted, no
+                                      effect (no fetch buffer in this engine) */
+  uint32_t add_aas_start;       /* --add-aan-start: accepted, no effect */
+  uint32_t add_aas_interval;    /* --add-aas-interval: accepted, no effect (all-
+                                   input roots come a wave's width of positions
+                                   per step; NAPgroups one position per step) */
   uint64_t result_capacity;     /* --result-capacity (reports) */
   uint32_t window;              /* batched: positions per work item, 0 = auto */
   uint32_t spill_capacity;      /* items per wave beyond the LDS ring, 0 = auto */

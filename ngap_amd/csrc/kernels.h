@@ -106,10 +106,13 @@ constexpr uint32_t kBlockThreads = 256;
 
 // Batched kernel a launch of P runs: 0 k_batched (one open work item per
 // wave: per-work-item traces, or request runs without a last-byte map), 1
-// k_batched_stream, 2 k_expand (after the root pass).
+// k_batched_stream of engine runs, 2 k_expand (after the root pass), 3
+// k_batched_stream of request runs (last-byte map).
+constexpr int kBatchedKinds = 4;
 NGAP_HD inline int batched_kind(const KParams &P) {
   if (P.task_items || (!P.chunk_len && !P.last_map)) return 0;
-  return P.root_items ? 2 : 1;
+  if (P.root_items) return 2;
+  return P.chunk_len ? 1 : 3;
 }
 
 // LDS bytes per wave for the given capacities

@@ -617,6 +617,13 @@ struct LaneGroup {
   uint32_t gtag, rec_base, cls_base;
 };
 
+// Where expand() finds a position's chunk end: engine runs (chunks of
+// chunk_len bytes, a divisibility test valid for items of any work item),
+// request runs of the streaming kernel (the last-byte map), or decided at run
+// time (chunk_len, else the wave's one open work item's bounds).
+enum : int { kEndChunkLen = 0, kEndMap = 1, kEndRuntime = 2 };
+
+template <int End = kEndRuntime>
 NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
                      uint32_t &p, int &depth, const LaneGroup &lg) {
   const uint32_t p1 = p + 1;
@@ -628,21 +635,23 @@ NGAP_DEV void expand(const KParams &P, WaveState &W, bool &valid, uint32_t &v,
   const uint32_t busy = __popcll(wballot(valid));
   W.items += busy;
   // is p1 the last position of its chunk (no successors enabled after it)?
-  // Engine runs have chunks of chunk_len bytes: a divisibility test, valid
-  // for items of any work item; request runs read the last-byte map (any
-  // request) or use the wave's one open work item's bounds.
-  const uint32_t lw = P.last_map ? P.last_map[valid ? p1 >> 5 : 0u] : 0u;
-  const bool last = P.chunk_len  ? (uint64_t)(p1 + 1) * P.chunk_magic <= P.chunk_magic - 1
-                    : P.last_map ? ((lw >> (p1 & 31)) & 1u) != 0u
-                                 : p1 + 1 >= W.cend;
+  bool last;
+  if (End == kEndMap) {
+    const uint32_t lw = P.last_map[valid ? p1 >> 5 : 0u];
+    last = ((lw >> (p1 & 31)) & 1u) != 0u;
+  } else if (End == kEndChunkLen || P.chunk_len) {
+    last = (uint64_t)(p1 + 1) * P.chunk_magic <= P.chunk_magic - 1;
+  } else {
+    last = p1 + 1 >= W.cend;
+  }
   bool has_next = false;
   uint32_t nv = 0;
-  bool more = valid;
   // work privatization (kernel_ngap_O4.cu:115-122): a matched successor is
   // expanded by the same lane while more than active_threshold lanes are
   // busy, and otherwise queued for the idle lanes
   const bool can_priv = (P.priv_limit < 0 || depth < P.priv_limit) &&
                         (int32_t)busy > P.active_threshold;
+  bool more = valid;
   while (wballot(more)) {
     int32_t link = -1;
     const int32_t es[4] = {rec.ids.x, rec.ids.y, rec.ids.z, rec.ids.w};
@@ -910,7 +919,7 @@ NGAP_DEV void take_roots(const KParams &P, WaveState &W, uint64_t from, uint32_t
 
 // drain() over a TaskStream: items of several work items (and groups) share
 // the queue, so a lane looks up its item's group bases when it takes one.
-template <bool RootItems>
+template <bool RootItems, int End>
 NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
   TaskStream src;
   src.start(P, W);
@@ -962,7 +971,7 @@ NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
       }
       continue;
     }
-    expand(P, W, valid, v, p, depth, lg);
+    expand<End>(P, W, valid, v, p, depth, lg);
   }
 }
 
@@ -1060,9 +1069,11 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
 
-// The batched engine for engine runs without per-work-item traces (chunks
-// of chunk_len bytes): each wave overlaps its work items (drain_stream).  A
-// kernel of its own, so its registers are not sized for the per-item path.
+// The batched engine without per-work-item traces: each wave overlaps its
+// work items (drain_stream).  Kernels of their own, so their registers are
+// not sized for the per-item path: engine runs (End = kEndChunkLen) and
+// request runs (kEndMap).
+template <int End>
 __global__ void __launch_bounds__(kBlockThreads) k_batched_stream(KParams P) {
   extern __shared__ __align__(16) unsigned char lds[];
   const uint32_t wave = threadIdx.x / kWave;
@@ -1071,7 +1082,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched_stream(KParams P) {
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
   clear_dedup(P, W);
-  drain_stream<false>(P, W);
+  drain_stream<false, End>(P, W);
   flush_reports(P, W);
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
@@ -1086,7 +1097,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_expand(KParams P) {
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
   clear_dedup(P, W);
-  drain_stream<true>(P, W);
+  drain_stream<true, kEndChunkLen>(P, W);
   flush_reports(P, W);
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
@@ -1294,21 +1305,39 @@ hipError_t launch_roots(hipStream_t s, const KParams &P, uint32_t lds) {
   return hipGetLastError();
 }
 
+// The kernel of each batched kind (batched_kind).
+static const void *batched_kernel(int kind) {
+  switch (kind) {
+    case 1: return reinterpret_cast<const void *>(k_batched_stream<kEndChunkLen>);
+    case 2: return reinterpret_cast<const void *>(k_expand);
+    case 3: return reinterpret_cast<const void *>(k_batched_stream<kEndMap>);
+    default: return reinterpret_cast<const void *>(k_batched);
+  }
+}
+
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds) {
   const int kind = batched_kind(P);
-  if (kind == 2)
-    hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(kBlockThreads), lds, s, P);
-  else if (kind == 1)
-    hipLaunchKernelGGL(k_batched_stream, dim3(blocks), dim3(kBlockThreads), lds, s, P);
-  else
-    hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+  switch (kind) {
+    case 1:
+      hipLaunchKernelGGL((k_batched_stream<kEndChunkLen>), dim3(blocks), dim3(kBlockThreads),
+                         lds, s, P);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_batched_stream<kEndMap>), dim3(blocks), dim3(kBlockThreads), lds,
+                         s, P);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);
+  }
   return hipGetLastError();
 }
 
 hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, int kind) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, kind == 2 ? k_expand : kind == 1 ? k_batched_stream : k_batched,
-      kBlockThreads, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_kernel(kind),
+                                                      kBlockThreads, lds);
 }
 
 hipError_t set_lds_limit(uint32_t lds) {
@@ -1318,9 +1347,11 @@ hipError_t set_lds_limit(uint32_t lds) {
   e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_blocking),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched_stream),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  if (e != hipSuccess) return e;
+  for (int kind = 1; kind <= kBatchedKinds - 1; kind++) {
+    e = hipFuncSetAttribute(batched_kernel(kind), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds);
+    if (e != hipSuccess) return e;
+  }
   e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_roots),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (e != hipSuccess) return e;

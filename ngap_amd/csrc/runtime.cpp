@@ -61,6 +61,8 @@ int RunCtx::init() {
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
   HIP_TRY(counters.reserve(128 + kQueues * kQueueStride * 4));
+  // the reset kernel clears the counters a run uses; the rest stays zero
+  HIP_TRY(hipMemset(counters.p, 0, counters.n));
   HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&host_cnt), 32, hipHostMallocDefault));
   return NGAP_OK;
 }
@@ -194,7 +196,7 @@ int engine_kind(const ngap_run_opts &o) { return o.task_traces ? 0 : o.root_pass
 
 int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *blocks,
                       uint32_t ring, int kind) {
-  const int slot = 3 * (31 - __builtin_clz(ring)) + kind;  // log2, ring in [64, 2048]
+  const int slot = kBatchedKinds * (31 - __builtin_clz(ring)) + kind;  // ring in [64, 2048]
   int per_cu = eng->batched_per_cu[slot];
   if (per_cu == 0) {
     HIP_TRY(batched_occupancy(&per_cu, lds_bytes(ring), kind));
@@ -353,7 +355,7 @@ int run_tasks(ngap_engine *eng, RunCtx &ctx, KParams P, const std::vector<Task> 
   if (rc) return rc;
   std::vector<uint32_t> bad;
   bool spill_over = false;  // some work item lost items (not only its root area)
-  if (cnt[3] && T) {  // per-item flags only when some work item overflowed
+  if ((uint32_t)cnt[3] && T) {  // per-item flags only when some work item overflowed
     std::vector<uint32_t> flags(T);
     HIP_TRY(hipMemcpy(flags.data(), P.overflow, T * 4ull, hipMemcpyDeviceToHost));
     bool root_over = false;

@@ -101,7 +101,7 @@ hipError_t launch_plain(hipStream_t s, const KParams &P, uint32_t units, uint32_
 hipError_t launch_blocking(hipStream_t s, const KParams &P, uint32_t units, uint32_t lds);
 hipError_t launch_roots(hipStream_t s, const KParams &P, uint32_t lds);
 hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint32_t lds);
-hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, int kind);  // 0 k_batched, 1 k_batched_stream, 2 k_expand
+hipError_t batched_occupancy(int *blocks_per_cu, uint32_t lds, int kind);  // kind: batched_kind()
 hipError_t set_lds_limit(uint32_t lds);
 
 }  // namespace ngapamd
@@ -109,7 +109,8 @@ hipError_t set_lds_limit(uint32_t lds);
 struct ngap_engine {
   int device = 0;
   int num_cus = 0;
-  int batched_per_cu[36] = {0};  // blocks per CU by (log2(ring), kernel kind), 0 = not queried
+  // blocks per CU by (log2(ring), batched kernel kind), 0 = not queried
+  int batched_per_cu[12 * ngapamd::kBatchedKinds] = {0};
   ngapamd::Image img;
   bool loaded = false;
   ngapamd::DevBuf d_meta, d_succ4, d_cls, d_sod, d_mask1;

@@ -358,7 +358,8 @@ int retire_slot(ngap_server *s, Slot &sl) {
   unsigned long long c4[4] = {0, 0, 0, 0};
   memcpy(c4, sl.ctx.host_cnt, 32);
   const unsigned long long cnt = c4[0];
-  const bool redo = cnt > sl.ctx.report_cap || c4[3] != 0;
+  // overflowed work items: the low word of the fourth counter
+  const bool redo = cnt > sl.ctx.report_cap || (uint32_t)c4[3] != 0;
   if (redo) {
     // overflow: re-run the whole launch synchronously with room for every
     // report and the engine's overflow fallback for the worklists (on the
@@ -443,7 +444,7 @@ int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
   if (!ring_of(*opts))
     return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
   // request launches run the streaming kernel (chunk ends from the last-byte map)
-  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts), 1);
+  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts), 3);
   if (rc) return rc;
   // every launch gets the whole persistent grid: launches in flight share the
   // CUs, and a lone launch is not held to a fraction of the GPU
