@@ -192,6 +192,28 @@ int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
 
 int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats);
 
+/* ngap_run without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The input
+ * (num_seg chunks of chunk_len bytes) and the report buffer (report_cap
+ * entries) are the caller's device memory; the run's counters, work-item
+ * flags and worklist spill areas live in a fixed 48 MiB device array of the
+ * library (one per device, held until the process ends), and its work items
+ * are generated on the device (the static schedule's shape: all-group
+ * windows at depth >= 2 with several groups, else one group per work item;
+ * opts->window or about two work items per wave).  Runs the batched
+ * streaming engine on the engine's stream and waits for it.  Reports are
+ * written raw in the device encoding ((offset << 32) | (state | group << 22)),
+ * unordered and possibly repeated; sorting, removing repeats and swapping
+ * the halves gives ngap_get_reports' output.  stats->reports_raw is their
+ * count.  Errors: more than 1M work items (use a larger window), a grid
+ * whose worklists do not fit the array, NGAP_ERR_OVERFLOW when a worklist
+ * (no fallback here) or report_cap overflows.  Calls that overlap on one
+ * device race on the shared array. */
+int ngap_run_noalloc(ngap_engine *eng, const ngap_run_opts *opts, const uint8_t *d_input,
+                     uint64_t chunk_len, uint32_t num_seg, uint64_t *d_reports,
+                     uint64_t report_cap, ngap_run_stats *stats);
+
 /* Autotuner of the batched engine (the reference picks these knobs by hand
  * per application, code/scripts/configs/exec_config_ngap_groups_best): times
  * candidate work-item windows, LDS worklist capacities, persistent grid sizes,

@@ -133,6 +133,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_uint64)]),
     "ngap_set_input": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32]),
     "ngap_run": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
+    "ngap_run_noalloc": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.c_void_p, C.c_uint64,
+                                   C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(RunStats)]),
     "ngap_autotune": (C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.c_int32, C.POINTER(RunOpts),
                                 C.POINTER(C.c_double)]),
     "ngap_batched_grid": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),

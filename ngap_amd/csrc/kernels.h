@@ -79,7 +79,13 @@ struct KParams {
   uint32_t *overflow;      // plain: per unit; batched: per task
   uint32_t *overflow_count;  // units / tasks that set their overflow flag
   unsigned long long *items;   // expanded items (statistic)
-  const Task *tasks;
+  const Task *tasks;        // nullable: work items generated from their index
+  // Generated work items (tasks == nullptr): chunks cut into windows of
+  // gen_window positions; all-group work items (gen_cross) ordered by
+  // (chunk, window), else one group per work item ordered by (chunk, group,
+  // window).
+  uint32_t gen_window;
+  int32_t gen_cross;
   const uint32_t *task_order;  // nullable
   uint32_t num_tasks;
   uint32_t *task_head;     // kQueues claim counters, kQueueStride words apart
@@ -104,15 +110,34 @@ constexpr uint32_t kMaxOpen = 8;  // work items a wave overlaps (k_batched)
 constexpr uint32_t kQueues = 8, kQueueStride = 32;
 constexpr uint32_t kBlockThreads = 256;
 
+// Work item t of a generated schedule (KParams::gen_window).
+NGAP_HD inline Task gen_task(const KParams &P, uint32_t t) {
+  const uint32_t L = P.chunk_len, win = P.gen_window;
+  const uint32_t per_chunk = (L + win - 1) / win;
+  const uint32_t per_unit = P.gen_cross ? per_chunk : per_chunk * P.num_groups;
+  const uint32_t c = t / per_unit, r = t % per_unit;
+  const uint32_t g = P.gen_cross ? kAllGroups : r / per_chunk, w = r % per_chunk;
+  Task T;
+  T.group = g;
+  T.cstart = c * L;
+  T.cend = T.cstart + L;
+  T.begin = T.cstart + w * win;
+  T.end = T.begin + win < T.cend ? T.begin + win : T.cend;
+  T.flags = (w == 0 && (c == 0 || P.is_dup)) ? kTaskSeedStart : 0u;
+  return T;
+}
+
 // Batched kernel a launch of P runs: 0 k_batched (one open work item per
 // wave: per-work-item traces, or request runs without a last-byte map), 1
 // k_batched_stream of engine runs, 2 k_expand (after the root pass), 3
-// k_batched_stream of request runs (last-byte map).
-constexpr int kBatchedKinds = 4;
+// k_batched_stream of request runs (last-byte map), 4 k_batched_stream of
+// generated work items.
+constexpr int kBatchedKinds = 5;
 NGAP_HD inline int batched_kind(const KParams &P) {
   if (P.task_items || (!P.chunk_len && !P.last_map)) return 0;
   if (P.root_items) return 2;
-  return P.chunk_len ? 1 : 3;
+  if (!P.chunk_len) return 3;
+  return P.tasks ? 1 : 4;
 }
 
 // LDS bytes per wave for the given capacities

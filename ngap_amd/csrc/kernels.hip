@@ -843,14 +843,14 @@ struct TaskStream {
   bool cross_mask = false;  // LDS holds the cross-group report mask
   uint32_t q = 0;           // the wave's queue
   NGAP_DEV void start(const KParams &P, const WaveState &W) { q = blockIdx.x % kQueues; }
-  template <bool RootItems>
+  template <bool RootItems, bool Gen>
   NGAP_DEV int next(const KParams &P, WaveState &W, uint32_t &a, uint32_t &b) {
     if (cur >= end) {
       if (W.n_open >= kMaxOpen) return 1;
       const uint32_t t = claim_task(P, W, q);
       if (t >= P.num_tasks) return 2;
       const uint32_t ti = P.task_order ? P.task_order[t] : t;
-      const Task T = P.tasks[ti];
+      const Task T = Gen ? gen_task(P, ti) : P.tasks[ti];
       if (W.lane == 0) W.open[W.n_open] = ti;
       W.n_open++;
       if (RootItems) {  // the root pass wrote this work item's roots
@@ -919,7 +919,7 @@ NGAP_DEV void take_roots(const KParams &P, WaveState &W, uint64_t from, uint32_t
 
 // drain() over a TaskStream: items of several work items (and groups) share
 // the queue, so a lane looks up its item's group bases when it takes one.
-template <bool RootItems, int End>
+template <bool RootItems, int End, bool Gen>
 NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
   TaskStream src;
   src.start(P, W);
@@ -936,7 +936,7 @@ NGAP_DEV void drain_stream(const KParams &P, WaveState &W) {
     }
     if (qn < P.ring_cap / 2 && W.spill_n == 0 && more) {  // as in drain()
       uint32_t a, b;
-      const int st = src.template next<RootItems>(P, W, a, b);
+      const int st = src.template next<RootItems, Gen>(P, W, a, b);
       if (st == 0) {
         if (RootItems) take_roots(P, W, src.rbase + a, b - a);
         else if (src.cross) roots_cross(P, W, a, b);
@@ -1071,9 +1071,10 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
 
 // The batched engine without per-work-item traces: each wave overlaps its
 // work items (drain_stream).  Kernels of their own, so their registers are
-// not sized for the per-item path: engine runs (End = kEndChunkLen) and
-// request runs (kEndMap).
-template <int End>
+// not sized for the per-item path: engine runs (End = kEndChunkLen, work
+// items from P.tasks), request runs (kEndMap) and engine runs whose work
+// items are generated from their index (Gen).
+template <int End, bool Gen>
 __global__ void __launch_bounds__(kBlockThreads) k_batched_stream(KParams P) {
   extern __shared__ __align__(16) unsigned char lds[];
   const uint32_t wave = threadIdx.x / kWave;
@@ -1082,7 +1083,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched_stream(KParams P) {
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
   clear_dedup(P, W);
-  drain_stream<false, End>(P, W);
+  drain_stream<false, End, Gen>(P, W);
   flush_reports(P, W);
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
@@ -1097,7 +1098,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_expand(KParams P) {
   setup_wave(P, W, lds + 16 + wave * wbytes,
              P.spill + ((size_t)blockIdx.x * P.waves_per_block + wave) * P.spill_cap);
   clear_dedup(P, W);
-  drain_stream<true, kEndChunkLen>(P, W);
+  drain_stream<true, kEndChunkLen, false>(P, W);
   flush_reports(P, W);
   if (W.lane == 0) atomicAdd(P.items, W.items);
 }
@@ -1308,9 +1309,10 @@ hipError_t launch_roots(hipStream_t s, const KParams &P, uint32_t lds) {
 // The kernel of each batched kind (batched_kind).
 static const void *batched_kernel(int kind) {
   switch (kind) {
-    case 1: return reinterpret_cast<const void *>(k_batched_stream<kEndChunkLen>);
+    case 1: return reinterpret_cast<const void *>(k_batched_stream<kEndChunkLen, false>);
     case 2: return reinterpret_cast<const void *>(k_expand);
-    case 3: return reinterpret_cast<const void *>(k_batched_stream<kEndMap>);
+    case 3: return reinterpret_cast<const void *>(k_batched_stream<kEndMap, false>);
+    case 4: return reinterpret_cast<const void *>(k_batched_stream<kEndChunkLen, true>);
     default: return reinterpret_cast<const void *>(k_batched);
   }
 }
@@ -1319,15 +1321,19 @@ hipError_t launch_batched(hipStream_t s, const KParams &P, uint32_t blocks, uint
   const int kind = batched_kind(P);
   switch (kind) {
     case 1:
-      hipLaunchKernelGGL((k_batched_stream<kEndChunkLen>), dim3(blocks), dim3(kBlockThreads),
-                         lds, s, P);
+      hipLaunchKernelGGL((k_batched_stream<kEndChunkLen, false>), dim3(blocks),
+                         dim3(kBlockThreads), lds, s, P);
       break;
     case 2:
       hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(kBlockThreads), lds, s, P);
       break;
     case 3:
-      hipLaunchKernelGGL((k_batched_stream<kEndMap>), dim3(blocks), dim3(kBlockThreads), lds,
-                         s, P);
+      hipLaunchKernelGGL((k_batched_stream<kEndMap, false>), dim3(blocks), dim3(kBlockThreads),
+                         lds, s, P);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_batched_stream<kEndChunkLen, true>), dim3(blocks),
+                         dim3(kBlockThreads), lds, s, P);
       break;
     default:
       hipLaunchKernelGGL(k_batched, dim3(blocks), dim3(kBlockThreads), lds, s, P);

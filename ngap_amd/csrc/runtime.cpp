@@ -665,6 +665,8 @@ int ngap_create(int32_t device, ngap_engine **out) {
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   e->num_cus = prop.multiProcessorCount;
   HIP_TRY(set_lds_limit(lds_bytes(kMaxRing)));
+  rc = noalloc_attach(device, &e->scratch);  // this device's scratch array (noalloc.cpp)
+  if (rc) return rc;
   *out = e.release();
   return NGAP_OK;
 }

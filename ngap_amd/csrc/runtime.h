@@ -134,6 +134,8 @@ struct ngap_engine {
     std::vector<ngapamd::Task> tasks;
   } sched;
   uint64_t generation = 1;      // bumped by ngap_load_groups / ngap_set_input
+  void *scratch = nullptr;      // device address of this device's scratch array
+                                // (ngap_run_noalloc)
   ngapamd::ImageSizes img_sizes;  // bytes of each device image array
   bool built_on_device = false;   // the last ngap_load_groups built on the device
   double build_ms = 0;            // its wall time
@@ -160,4 +162,7 @@ int sorted_reports(RunCtx &ctx, std::vector<uint64_t> &out);
 int build_image_device(ngap_engine *eng, const HostGroup *groups, int32_t num_groups,
                        int32_t depth, hipStream_t s, Image &img, ImageSizes &sz,
                        std::string &err);
+// With `device` current: the device address of that device's copy of the
+// scratch array of ngap_run_noalloc (looked up once per device).
+int noalloc_attach(int device, void **out);
 }  // namespace ngapamd
