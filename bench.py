@@ -128,11 +128,12 @@ def sample_stats(groups, data, chunk, depth, max_chunks=32):
         reports=int(rep.size))
 
 
-def cpu_baseline(groups, data, chunk, depth, budget_s=20.0):
+def cpu_baseline(groups, data, chunk, depth, budget_s=25.0):
     """The oracle running the reference's CPU validator loop as written
     (every all-input state tested at every position, ngap.cu:2171-2268) on
     host threads over a bounded sample of whole-chunk prefixes: one unit per
-    (group, chunk), threads = min(units, usable CPUs, 16)."""
+    (group, chunk), threads = min(units, usable CPUs, 16); about `budget_s`
+    seconds, and at least 1 MiB of input where that takes under 40 s."""
     import oracle
     model, nproc, usable = cpu_info()
     S = len(data) // chunk
@@ -144,7 +145,11 @@ def cpu_baseline(groups, data, chunk, depth, budget_s=20.0):
     rate = L0 / max(1e-6, time.time() - t)  # input bytes/s of one thread over all groups
     picks = spread_chunks(S, 16)
     threads = host_threads(G * len(picks))
-    L = int(min(chunk, max(256, rate * threads * budget_s / len(picks))))
+    L = max(256, rate * threads * budget_s / len(picks))
+    mib = (1 << 20) / len(picks)
+    if L < mib and mib * len(picks) / (rate * threads) < 40.0:
+        L = mib
+    L = int(min(chunk, L))
     sample = np.concatenate([data[c * chunk: c * chunk + L] for c in picks])
     units = G * len(picks)
     threads = host_threads(units)
@@ -293,10 +298,12 @@ def end_to_end(eng, data, chunk, opts, steps):
 
 
 def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, check=True,
-            cpu=True):
+            cpu=True, only=None):
     """All engines on one workload, each timed `repeats` times (interleaved)
     over `steps` steps; returns (result dict, per-engine runs, groups, input,
-    chunk)."""
+    chunk).  With `only`, that engine alone and no end-to-end leg: its
+    launches are then the last ones of the matching kernel, what a profiled
+    run summarises."""
     from ngap_amd import Engine
     t = time.time()
     groups, test, train, chunk = workload(name)
@@ -330,17 +337,20 @@ def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, c
                                           **common)
     tuned_knobs = None
     if tune:
-        topts, tms = eng.autotune(max_runs=12, **common)
+        topts, tms = eng.autotune(max_runs=24, **common)
         topts.use_graph = 1
         engines["tuned"] = topts
         tuned_knobs = dict(window=int(topts.window), persistent_blocks=int(topts.persistent_blocks),
                            spill_capacity=int(topts.spill_capacity),
-                           ring_capacity=int(topts.ring_capacity), root_pass=int(topts.root_pass))
+                           ring_capacity=int(topts.ring_capacity), root_pass=int(topts.root_pass),
+                           active_threshold=int(topts.active_threshold))
     if learned_on:
         engines["learned"] = lopts
         if lpopts is not None:
             # the learned schedule of per-group work items, next to "pergroup"
             engines["learned_pergroup"] = lpopts
+    if only:
+        engines = {only: engines[only]}
     runs = {k: dict(walls=[], kerns=[]) for k in engines}
     for rep in range(repeats):
         for label, opts in engines.items():
@@ -361,6 +371,11 @@ def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, c
             f"items {st['items']}, tasks {st['num_tasks']}, fallback {st['fallback_tasks']}, "
             f"reports {r['reports'].size}")
     main_s = "learned" if learned_on else "static"
+    if only:
+        eng.close()
+        return dict(workload=name, engine=only, kernel_ms=runs[only]["kernel_ms"],
+                    wall_ms=runs[only]["wall_ms"], steps=steps, warmup=warmup,
+                    reports=int(runs[only]["reports"].size)), runs, groups, test, chunk
     e2e = end_to_end(eng, test, chunk, engines[main_s], max(3, steps // 2))
     log(f"{name} end to end ({main_s}): {e2e}")
     eng.close()
@@ -484,6 +499,10 @@ def main():
                     help="run only the serving sweep (prints its JSON object)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle work counts")
+    ap.add_argument("--engine", default=None,
+                    choices=["plain", "static", "pergroup", "tuned", "learned", "learned_pergroup"],
+                    help="time this engine alone on --workload (no other engine, workload or "
+                         "end-to-end leg) and print its timing: the command to profile")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     a = ap.parse_args()
@@ -498,6 +517,12 @@ def main():
     if a.serving_only:
         print(json.dumps({"serving": serving_bench([float(r) for r in a.serving.split(",")],
                                                    n_requests=a.requests)}), flush=True)
+        return
+    if a.engine:
+        res = compare(a.workload, a.depth, a.steps, a.warmup,
+                      learned_on=a.engine.startswith("learned"), tune=a.engine == "tuned",
+                      repeats=1, check=False, cpu=False, only=a.engine)[0]
+        print(json.dumps({"profile": res}), flush=True)
         return
     res, runs, groups, data, chunk = compare(a.workload, a.depth, a.steps, a.warmup,
                                              learned_on=not a.no_learned, tune=not a.no_tune,

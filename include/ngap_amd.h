@@ -217,10 +217,12 @@ int ngap_run_noalloc(ngap_engine *eng, const ngap_run_opts *opts, const uint8_t 
 /* Autotuner of the batched engine (the reference picks these knobs by hand
  * per application, code/scripts/configs/exec_config_ngap_groups_best): times
  * candidate work-item windows, LDS worklist capacities, persistent grid sizes,
- * worklist spill capacities and the root pass (on or off) on the loaded input
- * by coordinate descent (at most max_runs timed runs) and writes the fastest
- * into *best: a copy of *base, batched, with fallback on, window /
- * ring_capacity / persistent_blocks / spill_capacity / root_pass set.  Report
+ * worklist spill capacities, the root pass (on or off) and the active
+ * threshold (the reference's tuning values 0, 8, 16, 24, 32, ngap.cu:1999) on
+ * the loaded input by coordinate descent (at most max_runs timed runs; 23
+ * cover every axis) and writes the fastest into *best: a copy of *base,
+ * batched, with fallback on, window / ring_capacity / persistent_blocks /
+ * spill_capacity / root_pass / active_threshold set.  Report
  * sets do not depend on these knobs; the chunk length is part of the input's
  * meaning (matches do not cross chunks) and is not tuned. */
 int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,

@@ -473,13 +473,16 @@ int main(int argc, char **argv) {
         desc = "schedule=plain";
       } else {
         double ms = 0;
-        if (ngap_autotune(eng, &ro, 24, &t, &ms) != NGAP_OK) {
+        if (ngap_autotune(eng, &ro, 23, &t, &ms) != NGAP_OK) {
           printf("[Tuning] autotuner failed: %s\n", ngap_last_error());
           continue;
         }
         desc = "schedule=batched, window=" + std::to_string(t.window) +
                ", persistent_blocks=" + std::to_string(t.persistent_blocks) +
-               ", spill_capacity=" + std::to_string(t.spill_capacity);
+               ", spill_capacity=" + std::to_string(t.spill_capacity) +
+               ", ring_capacity=" + std::to_string(t.ring_capacity) +
+               ", root_pass=" + std::to_string(t.root_pass) +
+               ", active_threshold=" + std::to_string(t.active_threshold);
       }
       printf("[Tuning] Try choice: \n   %s\n", desc.c_str());
       bool pass = false;

@@ -31,6 +31,7 @@ constexpr uint32_t kMinRing = 64, kMaxRing = 2048;  // batched ring_capacity ran
 struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
+  bool owned = true;  // false: a view into another buffer (the image arena)
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
@@ -117,6 +118,8 @@ struct ngap_engine {
   ngapamd::DevBuf d_beg[ngapamd::kMaxMemoDepth], d_nrep[ngapamd::kMaxMemoDepth],
       d_list[ngapamd::kMaxMemoDepth], d_sel[ngapamd::kMaxMemoDepth];
   ngapamd::DevBuf d_xr1, d_xe1, d_xmask1, d_xr2, d_xe2, d_xsod, d_xs3, d_xp3, d_x2rep;
+  // one allocation holding every image array above (they are views into it)
+  ngapamd::DevBuf d_arena;
   ngapamd::DevBuf d_input;
   uint64_t chunk_len = 0;
   uint32_t num_seg = 0;

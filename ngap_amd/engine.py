@@ -132,8 +132,9 @@ class Engine:
         return b.value * w.value
 
     def autotune(self, max_runs=16, **kw):
-        """Batched-engine knobs (window, persistent blocks, spill capacity)
-        timed on the loaded input; returns (options, best kernel ms)."""
+        """Batched-engine knobs (window, LDS ring, persistent blocks, spill
+        capacity, root pass, active threshold) timed on the loaded input;
+        returns (options, best kernel ms)."""
         base = self.options(**kw)
         best = _lib.RunOpts()
         ms = C.c_double()

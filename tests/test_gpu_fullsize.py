@@ -121,3 +121,34 @@ def test_learned_workload_full_size_every_chunk(engine):
                          dict(schedule="batched", precompute_depth=2, cross_groups=0),
                          dict(schedule="batched", precompute_depth=3, root_pass=True)],
                 want, "learned full")
+
+
+def test_serving_workload_full_size(engine):
+    """The serving workload's 64 resident automata (1k-200k states, 2.5M in
+    all) at depth 3 and the first 64 of its requests (1 KiB-4 MiB, the largest
+    3.8 MiB), with pattern instances planted at 1e-3 per byte, queued at once:
+    every request's reports equal the oracle's on that request alone."""
+    from ngap_amd.serve import Server
+    groups = synth.serving_groups()
+    say("serving workload: load", sum(g.num_states for g in groups), "states")
+    engine.load_groups(groups, 3)
+    rng = np.random.default_rng(2005)
+    reqs = []
+    for _, g, d in synth.serving_requests(seed=2004, n=64):
+        d = d.copy()
+        synth.plant(rng, d, [groups[g]], 1e-3)
+        reqs.append((g, d))
+    assert max(len(d) for _, d in reqs) > (3 << 20)
+    srv = Server(engine, max_inflight=4, algorithm="nonblockingallgroups", precompute_depth=3,
+                 unique=True)
+    say("serving workload: 64 requests at once")
+    ids = [srv.submit(g, d) for g, d in reqs]
+    got = srv.drain(ids)
+    srv.close()
+    total = 0
+    for i, (g, d) in enumerate(reqs):
+        want, _ = oracle.run([groups[g]], d, len(d), 1)
+        want = want | np.uint64(g << 22 << 32)
+        total += want.size
+        assert np.array_equal(got[ids[i]][0], want), (i, g, len(d))
+    assert total > 1000

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from ngap_amd import NgapError, synth
+from ngap_amd import learned, NgapError, synth
 from ngap_amd.groups import make_group
 from ngap_amd.serve import Server, run_poisson
 
@@ -234,6 +234,14 @@ def test_overflow_fallback_restores_exact_reports(engine):
     st = engine.run(algorithm="nonblockinggroups", schedule="batched", spill_capacity=64,
                     window=1024, fallback=True)
     assert st["fallback_tasks"] > 0
+    assert np.array_equal(engine.reports(), want)
+    # the caller's work items (as a learned schedule passes them) with
+    # worklists sized too small: the same fallback, the same reports
+    say("fallback, caller work items")
+    tasks = learned.fixed_tiling(1, 4, len(data) // 4, window=512)
+    st = engine.run(algorithm="nonblockinggroups", schedule="batched", spill_capacity=64,
+                    ring_capacity=64, tasks=tasks, fallback=True)
+    assert st["fallback_tasks"] > 0 and st["num_tasks"] == len(tasks)
     assert np.array_equal(engine.reports(), want)
 
 
