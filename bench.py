@@ -416,6 +416,48 @@ def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, c
     return res, runs, groups, test, chunk
 
 
+ALGORITHMS = ["blockinggroups", "NAPgroups", "nonblockinggroups", "nonblockingr1groups",
+              "nonblockingr2groups", "nonblockingpcgroups", "nonblockingallgroups"]
+
+
+def breakdown(name, depth, steps, warmup):
+    """Every --algorithm on one workload, with the reference's decomposition
+    (plain: one workgroup per (group, chunk) unit) and with the batched engine
+    (blockinggroups: plain only): the reference evaluation's breakdown of its
+    optimisations (README.md:141-150) re-measured on this engine.  Every
+    report set is compared with the first one."""
+    from ngap_amd import Engine, NgapError
+    groups, test, _, chunk = workload(name)
+    total = (len(test) // chunk) * chunk
+    eng = Engine(0)
+    eng.load_groups(groups, max(depth, 1))
+    eng.set_input(test, chunk)
+    rows, ref = [], None
+    for algo in ALGORITHMS:
+        for sched in ("plain", "batched"):
+            if algo == "blockinggroups" and sched == "batched":
+                continue
+            opts = eng.options(algorithm=algo, schedule=sched, precompute_depth=depth,
+                               unique=True, use_graph=sched == "batched", fallback=True)
+            try:
+                wall, kern, st = timed(eng, opts, steps, warmup)
+            except NgapError as e:  # e.g. a frontier beyond the blocking engine's capacity
+                rows.append(dict(algorithm=algo, schedule=sched, error=str(e)))
+                log(f"breakdown {name}: {rows[-1]}")
+                continue
+            rep = eng.reports()
+            if ref is None:
+                ref = rep
+            rows.append(dict(algorithm=algo, schedule=sched,
+                             mbps=round(total / (wall * 1e-3) / 1e6, 1),
+                             kernel_ms=round(kern, 4), items=int(st["items"]),
+                             bit_exact=bool(np.array_equal(rep, ref))))
+            log(f"breakdown {name}: {rows[-1]}")
+    eng.close()
+    return dict(workload=name, input_bytes=int(total), groups=len(groups),
+                precompute_depth=depth, steps=steps, rows=rows)
+
+
 def serving_bench(rates, n_requests=1024, depth=3):
     """Serving workload: 64 resident automata (1k..200k states), Poisson
     requests of 1 KiB..4 MiB Zipf(1.1) bytes; p50/p99 latency and aggregate
@@ -503,6 +545,10 @@ def main():
                     choices=["plain", "static", "pergroup", "tuned", "learned", "learned_pergroup"],
                     help="time this engine alone on --workload (no other engine, workload or "
                          "end-to-end leg) and print its timing: the command to profile")
+    ap.add_argument("--breakdown", default=None, metavar="WORKLOAD",
+                    choices=["learned", "batch1024", "regex20k", "literal64"],
+                    help="only time every --algorithm on WORKLOAD, plain and batched "
+                         "(prints its JSON object)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     a = ap.parse_args()
@@ -514,6 +560,10 @@ def main():
                           "(the (group, chunk) units would shard across GPUs with no exchange)"}))
         sys.exit(2)
 
+    if a.breakdown:
+        print(json.dumps({"breakdown": breakdown(a.breakdown, a.depth, a.steps, a.warmup)}),
+              flush=True)
+        return
     if a.serving_only:
         print(json.dumps({"serving": serving_bench([float(r) for r in a.serving.split(",")],
                                                    n_requests=a.requests)}), flush=True)
