@@ -181,6 +181,12 @@ def copy_calibration(nbytes, reps):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+# Batched engines run with --active-threshold 0, one of the reference's tuning
+# values (ngap.cu:1999) and the one the autotuner picks on the 1,024-automaton
+# workloads; the plain engine keeps the reference default.
+BATCHED_THRESHOLD = 0
+
+
 def learned_schedule(eng, groups, train, test, chunk, depth):
     """Trains the frontier-size MLP on traces of `train` (at most 10^6
     (group, 4 KiB window) samples) and shapes the batched schedule of `test`
@@ -213,7 +219,9 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
     # exists (several groups, depth >= 2): the headline schedule
     ptasks, pcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves)
     if depth >= 2 and len(groups) > 1:
-        tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, waves)
+        # 1.5 work items per wave: more items cost more than their balance
+        # saves, fewer leave waves idle
+        tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, waves, per_wave=1.5)
     else:
         tasks, tcost = ptasks, pcost
     spill = learned.spill_capacity(pred, tcost)
@@ -230,12 +238,14 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
     m.close()
     opts = eng.options(schedule="batched", tasks=tasks, spill_capacity=spill, fallback=True,
                        use_graph=True, algorithm="nonblockingallgroups",
-                       precompute_depth=depth, unique=True)
+                       precompute_depth=depth, unique=True,
+                       active_threshold=BATCHED_THRESHOLD)
     popts = None
     if depth >= 2 and len(groups) > 1:
         popts = eng.options(schedule="batched", tasks=ptasks, spill_capacity=pspill,
                             fallback=True, use_graph=True, algorithm="nonblockingallgroups",
-                            precompute_depth=depth, unique=True)
+                            precompute_depth=depth, unique=True,
+                            active_threshold=BATCHED_THRESHOLD)
         info["pergroup_tasks"] = int(len(ptasks))
     return opts, popts, info
 
@@ -330,11 +340,12 @@ def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, c
     else:
         eng.set_input(test, chunk)
     engines["plain"] = eng.options(schedule="plain", **common)
-    engines["static"] = eng.options(schedule="batched", use_graph=True, **common)
+    engines["static"] = eng.options(schedule="batched", use_graph=True,
+                                    active_threshold=BATCHED_THRESHOLD, **common)
     if len(groups) > 1 and depth >= 2:
         # batched, one group per work item (no cross-group index)
         engines["pergroup"] = eng.options(schedule="batched", use_graph=True, cross_groups=0,
-                                          **common)
+                                          active_threshold=BATCHED_THRESHOLD, **common)
     tuned_knobs = None
     if tune:
         topts, tms = eng.autotune(max_runs=24, **common)
