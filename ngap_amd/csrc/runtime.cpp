@@ -12,6 +12,7 @@
 #include "runtime.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -873,21 +874,43 @@ int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
   HIP_TRY(hipSetDevice(eng->device));
   size_t n = (size_t)chunk_len * num_seg;
   HIP_TRY(eng->d_input.reserve(n + 16));
-  HIP_TRY(hipMemcpy(eng->d_input.p, data, n, hipMemcpyHostToDevice));
-  // byte histogram for the static per-group schedule (four partial counts:
-  // no store-to-load dependence between neighbouring bytes)
-  uint32_t hist[4][256] = {{0}};
-  size_t i = 0;
-  for (; i + 4 <= n; i += 4) {
-    hist[0][data[i]]++;
-    hist[1][data[i + 1]]++;
-    hist[2][data[i + 2]]++;
-    hist[3][data[i + 3]]++;
+  // The input goes through two pinned blocks: host threads copy a block in
+  // (counting its bytes for the static per-group schedule on the way) while
+  // the other block's copy to the device runs.
+  InputStage &stg = eng->stage;
+  HIP_TRY(stg.init());
+  HostPool &pool = *stg.pool;
+  std::vector<std::array<uint64_t, 256>> hist(pool.width());
+  for (auto &h : hist) h.fill(0);
+  for (size_t off = 0, k = 0; off < n; off += InputStage::kBlock, k++) {
+    const size_t len = std::min(InputStage::kBlock, n - off);
+    uint8_t *st = stg.buf[k & 1];
+    if (k >= 2) HIP_TRY(hipEventSynchronize(stg.done[k & 1]));  // its last copy is done
+    const uint8_t *src = data + off;
+    pool.run_parts(len, [&](size_t part, size_t a, size_t b) {
+      std::copy(src + a, src + b, st + a);
+      // four partial counts: no store-to-load dependence between neighbours
+      uint32_t h[4][256] = {{0}};
+      size_t i = a;
+      for (; i + 4 <= b; i += 4) {
+        h[0][src[i]]++;
+        h[1][src[i + 1]]++;
+        h[2][src[i + 2]]++;
+        h[3][src[i + 3]]++;
+      }
+      for (; i < b; i++) h[0][src[i]]++;
+      for (int x = 0; x < 256; x++) hist[part][x] += (uint64_t)h[0][x] + h[1][x] + h[2][x] + h[3][x];
+    });
+    HIP_TRY(hipMemcpyAsync(eng->d_input.as<uint8_t>() + off, st, len, hipMemcpyHostToDevice,
+                           eng->ctx.stream));
+    HIP_TRY(hipEventRecord(stg.done[k & 1], eng->ctx.stream));
   }
-  for (; i < n; i++) hist[0][data[i]]++;
-  for (int b = 0; b < 256; b++)
-    eng->byte_freq[b] =
-        ((double)hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b]) / (double)n;
+  HIP_TRY(hipStreamSynchronize(eng->ctx.stream));
+  for (int b = 0; b < 256; b++) {
+    uint64_t c = 0;
+    for (auto &h : hist) c += h[b];
+    eng->byte_freq[b] = (double)c / (double)n;
+  }
   eng->chunk_len = chunk_len;
   eng->num_seg = num_seg;
   eng->have_input = true;

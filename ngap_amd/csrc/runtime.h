@@ -7,7 +7,10 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "../../include/ngap_amd.h"
+#include "host_pool.h"
 #include "kernels.h"
 #include "layout.h"
 
@@ -47,6 +50,35 @@ struct DevBuf {
   }
   template <class T>
   T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// Pinned staging of ngap_set_input: two blocks, so that host threads fill one
+// while the other's copy to the device runs.
+struct InputStage {
+  static constexpr size_t kBlock = 8u << 20;
+  uint8_t *buf[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};  // the block's copy to the device
+  std::unique_ptr<HostPool> pool;
+  InputStage() = default;
+  InputStage(const InputStage &) = delete;
+  InputStage &operator=(const InputStage &) = delete;
+  ~InputStage() {
+    for (int i = 0; i < 2; i++) {
+      if (done[i]) (void)hipEventSynchronize(done[i]);
+      if (buf[i]) (void)hipHostFree(buf[i]);
+      if (done[i]) (void)hipEventDestroy(done[i]);
+    }
+  }
+  hipError_t init() {
+    if (buf[0]) return hipSuccess;
+    for (int i = 0; i < 2; i++) {
+      hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&buf[i]), kBlock, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+      if (e != hipSuccess) return e;
+    }
+    pool.reset(new HostPool());
+    return hipSuccess;
+  }
 };
 
 // One stream with its own per-run buffers and captured graph.
@@ -121,6 +153,7 @@ struct ngap_engine {
   // one allocation holding every image array above (they are views into it)
   ngapamd::DevBuf d_arena;
   ngapamd::DevBuf d_input;
+  ngapamd::InputStage stage;    // pinned staging of ngap_set_input
   uint64_t chunk_len = 0;
   uint32_t num_seg = 0;
   bool have_input = false;

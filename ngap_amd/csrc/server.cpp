@@ -9,17 +9,14 @@
 // polling their events: reports are split back per request by offset range.
 // There is no counterpart in the reference (it runs one input per process).
 #include <algorithm>
-#include <atomic>
 #include <chrono>
-#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
-#include <mutex>
-#include <thread>
 #include <unordered_map>
 
+#include "host_pool.h"
 #include "runtime.h"
 
 using namespace ngapamd;
@@ -61,89 +58,6 @@ struct PinnedPool {
   ~PinnedPool() {
     for (uint8_t *p : all) (void)hipHostFree(p);
   }
-};
-
-// Host threads that share the staging copy of a large request with the
-// submitting thread: one thread's copy into pinned memory runs well below
-// what the host-to-device link takes.  One copy at a time (submit is called
-// from one thread); a copy returns when every part is done.
-class CopyPool {
- public:
-  static constexpr size_t kPart = 256u << 10;  // smallest part worth a thread
-  explicit CopyPool(unsigned helpers) {
-    for (unsigned i = 0; i < helpers; i++) th_.emplace_back([this] { loop(); });
-  }
-  ~CopyPool() {
-    {
-      std::lock_guard<std::mutex> g(m_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (auto &t : th_) t.join();
-  }
-  void copy(uint8_t *dst, const uint8_t *src, size_t n) {
-    const size_t parts = std::min<size_t>(th_.size() + 1, n / kPart);
-    if (parts <= 1) {
-      memcpy(dst, src, n);
-      return;
-    }
-    auto job = std::make_shared<Job>();
-    job->dst = dst;
-    job->src = src;
-    job->n = n;
-    job->step = (n / parts + 4095) & ~(size_t)4095;
-    job->parts = (n + job->step - 1) / job->step;
-    {
-      std::lock_guard<std::mutex> g(m_);
-      job_ = job;
-      gen_++;
-    }
-    cv_.notify_all();
-    run(*job);
-    std::unique_lock<std::mutex> l(m_);
-    done_cv_.wait(l, [&] { return job->done.load() == job->parts; });
-    job_.reset();
-  }
-
- private:
-  struct Job {
-    uint8_t *dst = nullptr;
-    const uint8_t *src = nullptr;
-    size_t n = 0, step = 0, parts = 0;
-    std::atomic<size_t> next{0}, done{0};
-  };
-  void run(Job &j) {
-    for (;;) {
-      const size_t i = j.next.fetch_add(1);
-      if (i >= j.parts) return;
-      const size_t a = i * j.step, b = std::min(j.n, a + j.step);
-      memcpy(j.dst + a, j.src + a, b - a);
-      if (j.done.fetch_add(1) + 1 == j.parts) {
-        std::lock_guard<std::mutex> g(m_);
-        done_cv_.notify_all();
-      }
-    }
-  }
-  void loop() {
-    uint64_t seen = 0;
-    for (;;) {
-      std::shared_ptr<Job> j;
-      {
-        std::unique_lock<std::mutex> l(m_);
-        cv_.wait(l, [&] { return stop_ || gen_ != seen; });
-        if (stop_) return;
-        seen = gen_;
-        j = job_;
-      }
-      if (j) run(*j);  // a job already finished has no parts left
-    }
-  }
-  std::vector<std::thread> th_;
-  std::mutex m_;
-  std::condition_variable cv_, done_cv_;
-  std::shared_ptr<Job> job_;
-  uint64_t gen_ = 0;
-  bool stop_ = false;
 };
 
 struct Request {
@@ -258,7 +172,7 @@ struct ngap_server {
   // statistics (ngap_server_stats)
   uint64_t peak_held = 0, peak_inflight = 0, launches = 0, rejected = 0, redone = 0;
   PinnedPool pinned;  // freed by ngap_server_destroy after every launch completed
-  std::unique_ptr<CopyPool> copier;  // staging copies of large requests
+  std::unique_ptr<HostPool> copier;  // staging copies of large requests
   uint64_t held() const {
     uint64_t h = 0;
     for (auto &x : slots) h += x->held();
@@ -540,9 +454,7 @@ int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
   // every launch gets the whole persistent grid: launches in flight share the
   // CUs, and a lone launch is not held to a fraction of the GPU
   s->blocks_per_launch = std::max<uint32_t>(1, blocks);
-  // up to 7 helpers: the copy of a 4 MiB request in 8 parts
-  const unsigned hw = std::thread::hardware_concurrency();
-  s->copier.reset(new CopyPool(std::min(7u, hw > 1 ? hw - 1 : 0u)));
+  s->copier.reset(new HostPool());  // the copy of a 4 MiB request in up to 8 parts
   for (int i = 0; i < max_inflight; i++) {
     s->slots.emplace_back(new Slot());
     rc = s->slots.back()->ctx.init();

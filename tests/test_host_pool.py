@@ -1,0 +1,18 @@
+"""The host thread pool behind the pinned staging copies (server submit and
+ngap_set_input), compiled on the CPU with g++."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_host_pool_parts_and_copies(tmp_path):
+    exe = tmp_path / "host_pool_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-pthread", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "host_pool_check.cpp")], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.startswith("PASS"), out.stdout + out.stderr
