@@ -219,9 +219,11 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
     # exists (several groups, depth >= 2): the headline schedule
     ptasks, pcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves)
     if depth >= 2 and len(groups) > 1:
-        # 1.5 work items per wave: more items cost more than their balance
-        # saves, fewer leave waves idle
-        tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, waves, per_wave=1.5)
+        # 1.5 work items per wave (more cost more than their balance saves,
+        # fewer leave waves idle), claimed in input order (neighbouring
+        # windows share their byte distribution, so the index lines they touch)
+        tasks, tcost = learned.cross_schedule_from_costs(cost, S, chunk, waves, per_wave=1.5,
+                                                         order="position")
     else:
         tasks, tcost = ptasks, pcost
     spill = learned.spill_capacity(pred, tcost)

@@ -186,7 +186,10 @@ int ngap_image_info(ngap_engine *eng, ngap_image_summary *info);
 int ngap_image_array(ngap_engine *eng, const char *name, void *out, uint64_t cap,
                      uint64_t *n_bytes);
 
-/* Uploads num_seg chunks of chunk_len bytes (the concatenated input). */
+/* Uploads num_seg chunks of chunk_len bytes (the concatenated input) and
+ * returns when they are on the device (data may be reused at once).  The
+ * bytes go through two 8 MiB pinned blocks of the engine, copied in by up to
+ * 8 host threads while the other block's copy to the device runs. */
 int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
                    uint32_t num_seg);
 

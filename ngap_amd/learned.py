@@ -103,12 +103,14 @@ ALL_GROUPS = 0xFFFFFFFF
 
 
 def cross_schedule_from_costs(cost, num_seg, chunk, waves, window=WINDOW, per_wave=2,
-                              min_piece=256):
+                              min_piece=256, order="cost"):
     """All-group work items (every group's roots in one window, precompute
     depth >= 2) shaped by per-(group, window) costs: a window's cost is the
     sum over groups, then windows are cut or merged as in
-    schedule_from_costs.  Returns work items ordered by decreasing cost and
-    their costs."""
+    schedule_from_costs.  Returns work items and their costs, ordered by
+    decreasing cost, or with order="position" by chunk and window: work items
+    claimed together then cover neighbouring input, whose byte distribution
+    (and so the index lines it touches) is alike."""
     K = windows_per_chunk(chunk, window)
     wcost = np.asarray(cost, np.float64).reshape(-1, num_seg * K).sum(0)
     target = max(wcost.sum() / max(1, waves * per_wave), 1e-9)
@@ -116,8 +118,12 @@ def cross_schedule_from_costs(cost, num_seg, chunk, waves, window=WINDOW, per_wa
     for c in range(num_seg):
         _tile(tasks, tcost, ALL_GROUPS, c, wcost[c * K:(c + 1) * K], chunk, window, target,
               min_piece)
-    order = np.argsort(-np.asarray(tcost), kind="stable")
-    return np.asarray(tasks, np.uint32)[order], np.asarray(tcost)[order]
+    tasks, tcost = np.asarray(tasks, np.uint32), np.asarray(tcost)
+    if order == "position":
+        idx = np.lexsort((tasks[:, 2], tasks[:, 1]))
+    else:
+        idx = np.argsort(-tcost, kind="stable")
+    return tasks[idx], tcost[idx]
 
 
 def schedule_from_costs(cost, num_groups, num_seg, chunk, waves, window=WINDOW,

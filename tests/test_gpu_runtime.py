@@ -33,11 +33,15 @@ def test_graph_replays_match_eager(engine):
     say("eager")
     t_eager = min(engine.run(**eager)["kernel_ms"] for _ in range(3))
     assert np.array_equal(engine.reports(), want)
+    st0 = engine.run(**eager)
     opts = engine.options(use_graph=True, **eager)
     for rep in range(4):
         say("graph replay", rep)
-        t_graph = engine.run(opts)["kernel_ms"]
+        st = engine.run(opts)
+        t_graph = st["kernel_ms"]
         assert np.array_equal(engine.reports(), want)
+        # the counter reset is part of the graph: every replay counts from zero
+        assert st["items"] == st0["items"] > 0 and st["reports_raw"] == st0["reports_raw"]
         # each replay is timed itself: not a stale event from an earlier launch
         assert 0 < t_graph < 4 * t_eager + 0.5, (t_graph, t_eager)
         # eager work between replays (a slower engine: a stale time would show)

@@ -91,3 +91,17 @@ def test_group_stats():
     assert np.isclose(s[0], np.log1p(3)) and np.isclose(s[1], np.log1p(3))
     assert np.isclose(s[2], np.log1p(1)) and s[3] == 0 and np.isclose(s[4], np.log1p(1))
     assert np.isclose(s[5], 1.0) and np.isclose(s[6], 1 / 3) and np.isclose(s[7], np.log1p(1))
+
+
+def test_cross_schedule_in_position_order():
+    rng = np.random.default_rng(6)
+    G, S, L, W = 4, 3, 4096 * 8, 4096
+    cost = rng.exponential(1.0, (G, S * 8))
+    by_cost, c1 = learned.cross_schedule_from_costs(cost, S, L, waves=4, window=W)
+    by_pos, c2 = learned.cross_schedule_from_costs(cost, S, L, waves=4, window=W,
+                                                   order="position")
+    key = lambda t: sorted(map(tuple, t.tolist()))
+    assert key(by_cost) == key(by_pos) and np.isclose(c1.sum(), c2.sum())
+    assert np.all(np.diff(by_pos[:, 1].astype(np.int64)) >= 0)
+    same_chunk = by_pos[1:, 1] == by_pos[:-1, 1]
+    assert np.all(by_pos[1:, 2][same_chunk] == by_pos[:-1, 3][same_chunk])
