@@ -216,9 +216,9 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
     waves = eng.batched_waves()
     cost = learned.predicted_cost(pred).reshape(len(groups), S * K)
     # per-group work items, and all-group ones where the cross-group index
-    # exists (several groups, depth >= 2): the headline schedule
+    # exists (depth >= 2, one group included): the headline schedule
     ptasks, pcost = learned.schedule_from_costs(cost, len(groups), S, chunk, waves)
-    if depth >= 2 and len(groups) > 1:
+    if depth >= 2:
         # 1.5 work items per wave (more cost more than their balance saves,
         # fewer leave waves idle), claimed in input order (neighbouring
         # windows share their byte distribution, so the index lines they touch)
@@ -243,7 +243,7 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
                        precompute_depth=depth, unique=True,
                        active_threshold=BATCHED_THRESHOLD)
     popts = None
-    if depth >= 2 and len(groups) > 1:
+    if depth >= 2:
         popts = eng.options(schedule="batched", tasks=ptasks, spill_capacity=pspill,
                             fallback=True, use_graph=True, algorithm="nonblockingallgroups",
                             precompute_depth=depth, unique=True,
@@ -259,7 +259,7 @@ def roofline_of(groups, data, chunk, depth, kern_ms, engine_label, workload, tra
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary when one exists for this
     workload and engine."""
     ss = sample_stats(groups, data, chunk, depth)
-    cross = depth >= 2 and len(groups) > 1
+    cross = depth >= 2
     model = ss["cross"] if cross else ss["per_group"]
     total = (len(data) // chunk) * chunk
     log(f"{workload} oracle sample: {ss['chunks']} whole chunks ({ss['positions']} bytes) in "
@@ -344,7 +344,7 @@ def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, c
     engines["plain"] = eng.options(schedule="plain", **common)
     engines["static"] = eng.options(schedule="batched", use_graph=True,
                                     active_threshold=BATCHED_THRESHOLD, **common)
-    if len(groups) > 1 and depth >= 2:
+    if depth >= 2:
         # batched, one group per work item (no cross-group index)
         engines["pergroup"] = eng.options(schedule="batched", use_graph=True, cross_groups=0,
                                           active_threshold=BATCHED_THRESHOLD, **common)

@@ -107,10 +107,11 @@ typedef struct {
   uint32_t num_tasks;
   int32_t fallback;             /* re-run overflowing work items with a larger
                                    worklist instead of failing */
-  int32_t cross_groups;         /* batched static schedule: 1 = work items span
-                                   all groups when there are several and the
-                                   precompute depth is >= 2, 2 = also for one
-                                   group, 0 = one group per work item */
+  int32_t cross_groups;         /* batched static schedule: 1 (or 2) = work
+                                   items span all groups when the precompute
+                                   depth is >= 2 (one group too: its roots
+                                   then come from the 3-gram slots), 0 = one
+                                   group per work item */
   uint32_t ring_capacity;       /* batched: LDS worklist items per wave, a power
                                    of two in [64, 2048], 0 = 256 */
   int32_t task_traces;          /* batched: record the items of every work item
@@ -203,7 +204,7 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *stats)
  * flags and worklist spill areas live in a fixed 48 MiB device array of the
  * library (one per device, held until the process ends), and its work items
  * are generated on the device (the static schedule's shape: all-group
- * windows at depth >= 2 with several groups, else one group per work item;
+ * windows at depth >= 2, else one group per work item;
  * opts->window or about two work items per wave).  Runs the batched
  * streaming engine on the engine's stream and waits for it.  Reports are
  * written raw in the device encoding ((offset << 32) | (state | group << 22)),
