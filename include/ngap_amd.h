@@ -100,7 +100,9 @@ typedef struct {
   uint64_t result_capacity;     /* --result-capacity (reports) */
   uint32_t window;              /* batched: positions per work item, 0 = auto */
   uint32_t spill_capacity;      /* items per wave beyond the LDS ring, 0 = auto */
-  uint32_t persistent_blocks;   /* batched: workgroups, 0 = auto */
+  uint32_t persistent_blocks;   /* batched: workgroups, 0 = auto (the device's
+                                   occupancy, fewer for a job of fewer than
+                                   640 positions per wave) */
   int32_t use_graph;            /* batched: replay a captured hipGraph */
   const ngap_task *tasks;       /* batched: work items in run order (NULL = the
                                    engine's static schedule) */

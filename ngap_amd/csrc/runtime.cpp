@@ -209,6 +209,12 @@ int persistent_blocks(const ngap_engine *eng, uint32_t requested, uint32_t *bloc
   return NGAP_OK;
 }
 
+uint32_t job_blocks(uint32_t full_blocks, uint32_t waves_per_block, uint64_t work) {
+  const uint64_t per_block = kMinWavePositions * std::max<uint32_t>(1, waves_per_block);
+  const uint64_t want = (work + per_block - 1) / per_block;
+  return (uint32_t)std::min<uint64_t>(full_blocks, std::max<uint64_t>(64, want));
+}
+
 // Root pass set-up for the resident work items: each work item's area of
 // root items (factor items per position, the start-of-data seeds of a seeding
 // work item and 64 spare) and its pieces of at most kRootPiece positions.
@@ -924,7 +930,7 @@ int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
 int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   if (!eng || !opts) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
   if (st) memset(st, 0, sizeof *st);
-  const ngap_run_opts &o = *opts;
+  ngap_run_opts o = *opts;  // a copy: the default grid is resolved below
   if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
   if (!eng->have_input) return ngap_fail(NGAP_ERR_STATE, "no input set");
   HIP_TRY(hipSetDevice(eng->device));
@@ -955,6 +961,20 @@ int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
   // one group included: a 3-gram slot per position replaces the group's
   // rank-table lookups (regex20k: 0.185 ms against 0.217)
   const int kind = o.tasks ? 0 : (mp.memo_depth >= 2 && o.cross_groups != 0) ? 1 : 2;
+  // the default grid, sized to the job
+  if (o.persistent_blocks == 0) {
+    uint32_t full = 0;
+    int rc = persistent_blocks(eng, 0, &full, P.ring_cap, engine_kind(o));
+    if (rc) return rc;
+    uint64_t work = 0;
+    if (kind == 0) {
+      for (uint32_t i = 0; i < o.num_tasks; i++)
+        work += o.tasks[i].end > o.tasks[i].begin ? o.tasks[i].end - o.tasks[i].begin : 0;
+    } else {
+      work = (uint64_t)eng->num_seg * eng->chunk_len * (kind == 2 ? eng->img.num_groups : 1);
+    }
+    o.persistent_blocks = job_blocks(full, P.waves_per_block, work);
+  }
   uint32_t waves = 0;
   if (kind != 0) {
     uint32_t blocks = 0;
