@@ -11,9 +11,10 @@ literal64.
 A step is one pass of the engine over the whole input with inputs resident in
 HBM: counter reset + matching launch + overflow check (+ fallback re-runs).
 Engines timed on the same input: plain (one workgroup per (group, chunk)
-unit), batched with the static schedule (hipGraph replay), batched with the
-autotuned knobs, and batched with the learned schedule (the frontier-size MLP
-trained on traces of a second input from the same generator, different seed).
+unit), batched with the static schedule, batched one group per work item,
+batched with the autotuned knobs, and batched with the learned schedule (the
+frontier-size MLP trained on traces of a second input from the same
+generator, different seed), with and without its per-group shape.
 
 Printed JSON (one line): value = input MB/s of the batched + learned engine
 (wall clock over the timed steps), the other engines' MB/s and speedups, the
@@ -185,6 +186,10 @@ def copy_calibration(nbytes, reps):
 # values (ngap.cu:1999) and the one the autotuner picks on the 1,024-automaton
 # workloads; the plain engine keeps the reference default.
 BATCHED_THRESHOLD = 0
+# A run is one kernel launch: a direct launch costs about 8 us less per run
+# than replaying it from a captured graph (0.264 vs 0.272 ms wall per run on
+# the learned workload), so the bench launches directly.
+USE_GRAPH = False
 
 
 def learned_schedule(eng, groups, train, test, chunk, depth):
@@ -239,13 +244,13 @@ def learned_schedule(eng, groups, train, test, chunk, depth):
             "mae_constant_predictor": round(float(np.mean(np.abs(np.mean(y) - ty))), 4)}
     m.close()
     opts = eng.options(schedule="batched", tasks=tasks, spill_capacity=spill, fallback=True,
-                       use_graph=True, algorithm="nonblockingallgroups",
+                       use_graph=USE_GRAPH, algorithm="nonblockingallgroups",
                        precompute_depth=depth, unique=True,
                        active_threshold=BATCHED_THRESHOLD)
     popts = None
     if depth >= 2:
         popts = eng.options(schedule="batched", tasks=ptasks, spill_capacity=pspill,
-                            fallback=True, use_graph=True, algorithm="nonblockingallgroups",
+                            fallback=True, use_graph=USE_GRAPH, algorithm="nonblockingallgroups",
                             precompute_depth=depth, unique=True,
                             active_threshold=BATCHED_THRESHOLD)
         info["pergroup_tasks"] = int(len(ptasks))
@@ -342,16 +347,16 @@ def compare(name, depth, steps, warmup, learned_on=True, tune=True, repeats=5, c
     else:
         eng.set_input(test, chunk)
     engines["plain"] = eng.options(schedule="plain", **common)
-    engines["static"] = eng.options(schedule="batched", use_graph=True,
+    engines["static"] = eng.options(schedule="batched", use_graph=USE_GRAPH,
                                     active_threshold=BATCHED_THRESHOLD, **common)
     if depth >= 2:
         # batched, one group per work item (no cross-group index)
-        engines["pergroup"] = eng.options(schedule="batched", use_graph=True, cross_groups=0,
+        engines["pergroup"] = eng.options(schedule="batched", use_graph=USE_GRAPH, cross_groups=0,
                                           active_threshold=BATCHED_THRESHOLD, **common)
     tuned_knobs = None
     if tune:
         topts, tms = eng.autotune(max_runs=24, **common)
-        topts.use_graph = 1
+        topts.use_graph = int(USE_GRAPH)
         engines["tuned"] = topts
         tuned_knobs = dict(window=int(topts.window), persistent_blocks=int(topts.persistent_blocks),
                            spill_capacity=int(topts.spill_capacity),
@@ -451,7 +456,8 @@ def breakdown(name, depth, steps, warmup):
             if algo == "blockinggroups" and sched == "batched":
                 continue
             opts = eng.options(algorithm=algo, schedule=sched, precompute_depth=depth,
-                               unique=True, use_graph=sched == "batched", fallback=True)
+                               unique=True, use_graph=USE_GRAPH and sched == "batched",
+                               fallback=True)
             try:
                 wall, kern, st = timed(eng, opts, steps, warmup)
             except NgapError as e:  # e.g. a frontier beyond the blocking engine's capacity
