@@ -21,11 +21,20 @@ class HostPool {
  public:
   static constexpr size_t kPart = 256u << 10;  // smallest part worth a thread
 
-  // up to `max_helpers` helper threads, no more than the host has CPUs - 1
+  // up to `max_helpers` helper threads, no more than the host has CPUs - 1;
+  // a thread the system refuses leaves the pool with fewer (the caller's
+  // thread always runs parts itself)
   explicit HostPool(unsigned max_helpers = 7) {
     const unsigned hw = std::thread::hardware_concurrency();
     const unsigned n = std::min(max_helpers, hw > 1 ? hw - 1 : 0u);
-    for (unsigned i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+    th_.reserve(n);
+    for (unsigned i = 0; i < n; i++) {
+      try {
+        th_.emplace_back([this] { loop(); });
+      } catch (...) {
+        break;
+      }
+    }
   }
   HostPool(const HostPool &) = delete;
   HostPool &operator=(const HostPool &) = delete;

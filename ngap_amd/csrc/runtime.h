@@ -69,14 +69,26 @@ struct InputStage {
       if (done[i]) (void)hipEventDestroy(done[i]);
     }
   }
+  // allocates what an earlier, failed call left missing
   hipError_t init() {
-    if (buf[0]) return hipSuccess;
     for (int i = 0; i < 2; i++) {
-      hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&buf[i]), kBlock, hipHostMallocDefault);
-      if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
-      if (e != hipSuccess) return e;
+      if (!buf[i]) {
+        const hipError_t e =
+            hipHostMalloc(reinterpret_cast<void **>(&buf[i]), kBlock, hipHostMallocDefault);
+        if (e != hipSuccess) {
+          buf[i] = nullptr;
+          return e;
+        }
+      }
+      if (!done[i]) {
+        const hipError_t e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+        if (e != hipSuccess) {
+          done[i] = nullptr;
+          return e;
+        }
+      }
     }
-    pool.reset(new HostPool());
+    if (!pool) pool.reset(new HostPool());
     return hipSuccess;
   }
 };
