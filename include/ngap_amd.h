@@ -1,7 +1,9 @@
 /* ngap_amd — MI355X (gfx950) non-blocking automata processing, C ABI.
  *
  * Plain pointers and sizes only.  Every function returns an int status
- * (NGAP_OK = 0) and leaves a message for ngap_last_error() on failure.
+ * (NGAP_OK = 0) and leaves a message for ngap_last_error() on failure; no
+ * C++ exception crosses the ABI (a host allocation or thread failure inside a
+ * call returns NGAP_ERR_HOST).
  * One engine may be used by one host thread at a time.
  *
  * Reference interfaces replaced (paths relative to the reference tree):
@@ -39,6 +41,7 @@ extern "C" {
 #define NGAP_ERR_OOM 3         /* device allocation failed */
 #define NGAP_ERR_OVERFLOW 4    /* worklist or report capacity exceeded */
 #define NGAP_ERR_STATE 5       /* call out of order (no groups / no input) */
+#define NGAP_ERR_HOST 6        /* host memory or threads exhausted */
 
 /* --algorithm names (code/src/ngap/main.cu:114-130) */
 #define NGAP_ALGO_BLOCKING 0        /* blockinggroups */

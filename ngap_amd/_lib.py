@@ -15,6 +15,7 @@ NGAP_ERR_DEVICE = 2
 NGAP_ERR_OOM = 3
 NGAP_ERR_OVERFLOW = 4
 NGAP_ERR_STATE = 5
+NGAP_ERR_HOST = 6
 
 ALGORITHMS = {  # --algorithm names (code/src/ngap/main.cu:114-130)
     "blockinggroups": 0,

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/ngap_amd.h"
+#include "c_guard.h"
 
 void ngap_set_last_error_internal(const std::string &m);
 
@@ -284,171 +285,187 @@ int grad_on_device(ngap_mlp *m, uint64_t n) {
 extern "C" {
 
 int ngap_mlp_create(int32_t device, uint64_t seed, ngap_mlp **out) {
-  if (!out) return fail(NGAP_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
-    return fail(NGAP_ERR_DEVICE, "no HIP device available");
-  MLP_TRY(hipSetDevice(device));
-  MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_backward),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBackLds));
-  ngap_mlp *m = new ngap_mlp();
-  m->device = device;
-  std::vector<float> p(kParams, 0.f);
-  // He-uniform initialisation from a splitmix64 stream
-  uint64_t z = seed + 0x9E3779B97F4A7C15ull;
-  auto next = [&]() {
-    z += 0x9E3779B97F4A7C15ull;
-    uint64_t x = z;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    x ^= x >> 31;
-    return (double)(x >> 11) / 9007199254740992.0;
-  };
-  auto fill = [&](int off, int cnt, int fan_in) {
-    const double a = std::sqrt(6.0 / fan_in);
-    for (int i = 0; i < cnt; i++) p[off + i] = (float)((2 * next() - 1) * a);
-  };
-  fill(kW1, kIn * kHid, kIn);
-  fill(kW2, kHid * kHid, kHid);
-  fill(kW3, kHid, kHid);
-  int rc = NGAP_OK;
-  if (m->P.reserve(kParams * 4) || m->G.reserve(kParams * 4) || m->M.reserve(kParams * 4) ||
-      m->V.reserve(kParams * 4) || m->loss.reserve(4)) {
-    delete m;
-    return fail(NGAP_ERR_OOM, "MLP allocation failed");
-  }
-  if (hipMemcpy(m->P.p, p.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess) rc = 1;
-  std::vector<float> zero(kParams, 0.f);
-  if (hipMemcpy(m->G.p, zero.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(m->M.p, zero.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(m->V.p, zero.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess)
-    rc = 1;
-  if (rc) {
-    delete m;
-    return fail(NGAP_ERR_DEVICE, "MLP initialisation failed");
-  }
-  *out = m;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!out) return fail(NGAP_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      return fail(NGAP_ERR_DEVICE, "no HIP device available");
+    MLP_TRY(hipSetDevice(device));
+    MLP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(k_backward),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBackLds));
+    ngap_mlp *m = new ngap_mlp();
+    m->device = device;
+    std::vector<float> p(kParams, 0.f);
+    // He-uniform initialisation from a splitmix64 stream
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+    auto next = [&]() {
+      z += 0x9E3779B97F4A7C15ull;
+      uint64_t x = z;
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      x ^= x >> 31;
+      return (double)(x >> 11) / 9007199254740992.0;
+    };
+    auto fill = [&](int off, int cnt, int fan_in) {
+      const double a = std::sqrt(6.0 / fan_in);
+      for (int i = 0; i < cnt; i++) p[off + i] = (float)((2 * next() - 1) * a);
+    };
+    fill(kW1, kIn * kHid, kIn);
+    fill(kW2, kHid * kHid, kHid);
+    fill(kW3, kHid, kHid);
+    int rc = NGAP_OK;
+    if (m->P.reserve(kParams * 4) || m->G.reserve(kParams * 4) || m->M.reserve(kParams * 4) ||
+        m->V.reserve(kParams * 4) || m->loss.reserve(4)) {
+      delete m;
+      return fail(NGAP_ERR_OOM, "MLP allocation failed");
+    }
+    if (hipMemcpy(m->P.p, p.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess) rc = 1;
+    std::vector<float> zero(kParams, 0.f);
+    if (hipMemcpy(m->G.p, zero.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->M.p, zero.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->V.p, zero.data(), kParams * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = 1;
+    if (rc) {
+      delete m;
+      return fail(NGAP_ERR_DEVICE, "MLP initialisation failed");
+    }
+    *out = m;
+    return NGAP_OK;
+  });
 }
 
 int ngap_mlp_destroy(ngap_mlp *m) {
-  delete m;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    delete m;
+    return NGAP_OK;
+  });
 }
 
 int32_t ngap_mlp_num_params(void) { return kParams; }
 
 int ngap_mlp_get_params(ngap_mlp *m, float *out) {
-  if (!m || !out) return fail(NGAP_ERR_INVALID, "NULL argument");
-  MLP_TRY(hipMemcpy(out, m->P.p, kParams * 4, hipMemcpyDeviceToHost));
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!m || !out) return fail(NGAP_ERR_INVALID, "NULL argument");
+    MLP_TRY(hipMemcpy(out, m->P.p, kParams * 4, hipMemcpyDeviceToHost));
+    return NGAP_OK;
+  });
 }
 
 int ngap_mlp_set_params(ngap_mlp *m, const float *in) {
-  if (!m || !in) return fail(NGAP_ERR_INVALID, "NULL argument");
-  MLP_TRY(hipMemcpy(m->P.p, in, kParams * 4, hipMemcpyHostToDevice));
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!m || !in) return fail(NGAP_ERR_INVALID, "NULL argument");
+    MLP_TRY(hipMemcpy(m->P.p, in, kParams * 4, hipMemcpyHostToDevice));
+    return NGAP_OK;
+  });
 }
 
 int ngap_mlp_predict(ngap_mlp *m, const float *x, uint64_t n, float *y) {
-  if (!m || (!x && n) || (!y && n)) return fail(NGAP_ERR_INVALID, "NULL argument");
-  if (!n) return NGAP_OK;
-  MLP_TRY(hipSetDevice(m->device));
-  int rc = load_batch(m, x, nullptr, n);
-  if (rc) return rc;
-  rc = forward(m, n, false);
-  if (rc) return rc;
-  MLP_TRY(hipMemcpy(y, m->Y.p, n * 4, hipMemcpyDeviceToHost));
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!m || (!x && n) || (!y && n)) return fail(NGAP_ERR_INVALID, "NULL argument");
+    if (!n) return NGAP_OK;
+    MLP_TRY(hipSetDevice(m->device));
+    int rc = load_batch(m, x, nullptr, n);
+    if (rc) return rc;
+    rc = forward(m, n, false);
+    if (rc) return rc;
+    MLP_TRY(hipMemcpy(y, m->Y.p, n * 4, hipMemcpyDeviceToHost));
+    return NGAP_OK;
+  });
 }
 
 int ngap_mlp_predict_pairs(ngap_mlp *m, const float *hist, uint32_t num_windows,
                            const float *stats, uint32_t num_groups, const uint32_t *pairs,
                            uint64_t n, float *y) {
-  if (!m || !hist || !stats || !pairs || !y) return fail(NGAP_ERR_INVALID, "NULL argument");
-  for (uint64_t i = 0; i < n; i++)
-    if (pairs[2 * i] >= num_groups || pairs[2 * i + 1] >= num_windows)
-      return fail(NGAP_ERR_INVALID, "pair out of range");
-  if (!n) return NGAP_OK;
-  MLP_TRY(hipSetDevice(m->device));
-  MLP_TRY(m->hist.reserve((size_t)num_windows * 256 * 4));
-  MLP_TRY(m->stats.reserve((size_t)num_groups * 8 * 4));
-  MLP_TRY(m->pairs.reserve(n * 8));
-  MLP_TRY(m->Y.reserve(n * 4));
-  MLP_TRY(hipMemcpy(m->hist.p, hist, (size_t)num_windows * 256 * 4, hipMemcpyHostToDevice));
-  MLP_TRY(hipMemcpy(m->stats.p, stats, (size_t)num_groups * 8 * 4, hipMemcpyHostToDevice));
-  MLP_TRY(hipMemcpy(m->pairs.p, pairs, n * 8, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_forward, dim3(m->blocks), dim3(kThreads), 0, 0, m->P.f(), nullptr,
-                     m->hist.f(), m->stats.f(), reinterpret_cast<uint32_t *>(m->pairs.p), n,
-                     m->Y.f(), nullptr, nullptr);
-  MLP_TRY(hipGetLastError());
-  MLP_TRY(hipMemcpy(y, m->Y.p, n * 4, hipMemcpyDeviceToHost));
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!m || !hist || !stats || !pairs || !y) return fail(NGAP_ERR_INVALID, "NULL argument");
+    for (uint64_t i = 0; i < n; i++)
+      if (pairs[2 * i] >= num_groups || pairs[2 * i + 1] >= num_windows)
+        return fail(NGAP_ERR_INVALID, "pair out of range");
+    if (!n) return NGAP_OK;
+    MLP_TRY(hipSetDevice(m->device));
+    MLP_TRY(m->hist.reserve((size_t)num_windows * 256 * 4));
+    MLP_TRY(m->stats.reserve((size_t)num_groups * 8 * 4));
+    MLP_TRY(m->pairs.reserve(n * 8));
+    MLP_TRY(m->Y.reserve(n * 4));
+    MLP_TRY(hipMemcpy(m->hist.p, hist, (size_t)num_windows * 256 * 4, hipMemcpyHostToDevice));
+    MLP_TRY(hipMemcpy(m->stats.p, stats, (size_t)num_groups * 8 * 4, hipMemcpyHostToDevice));
+    MLP_TRY(hipMemcpy(m->pairs.p, pairs, n * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_forward, dim3(m->blocks), dim3(kThreads), 0, 0, m->P.f(), nullptr,
+                       m->hist.f(), m->stats.f(), reinterpret_cast<uint32_t *>(m->pairs.p), n,
+                       m->Y.f(), nullptr, nullptr);
+    MLP_TRY(hipGetLastError());
+    MLP_TRY(hipMemcpy(y, m->Y.p, n * 4, hipMemcpyDeviceToHost));
+    return NGAP_OK;
+  });
 }
 
 int ngap_mlp_gradient(ngap_mlp *m, const float *x, const float *t, uint64_t n, float *grad,
                       float *loss) {
-  if (!m || !x || !t || !grad || !loss || !n) return fail(NGAP_ERR_INVALID, "bad argument");
-  MLP_TRY(hipSetDevice(m->device));
-  int rc = load_batch(m, x, t, n);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_zero, dim3((kParams + 255) / 256), dim3(256), 0, 0, m->G.f(), kParams);
-  MLP_TRY(hipGetLastError());
-  rc = grad_on_device(m, n);
-  if (rc) return rc;
-  MLP_TRY(hipMemcpy(grad, m->G.p, kParams * 4, hipMemcpyDeviceToHost));
-  float l = 0;
-  MLP_TRY(hipMemcpy(&l, m->loss.p, 4, hipMemcpyDeviceToHost));
-  *loss = l / (float)n;
-  hipLaunchKernelGGL(k_zero, dim3((kParams + 255) / 256), dim3(256), 0, 0, m->G.f(), kParams);
-  MLP_TRY(hipGetLastError());
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!m || !x || !t || !grad || !loss || !n) return fail(NGAP_ERR_INVALID, "bad argument");
+    MLP_TRY(hipSetDevice(m->device));
+    int rc = load_batch(m, x, t, n);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_zero, dim3((kParams + 255) / 256), dim3(256), 0, 0, m->G.f(), kParams);
+    MLP_TRY(hipGetLastError());
+    rc = grad_on_device(m, n);
+    if (rc) return rc;
+    MLP_TRY(hipMemcpy(grad, m->G.p, kParams * 4, hipMemcpyDeviceToHost));
+    float l = 0;
+    MLP_TRY(hipMemcpy(&l, m->loss.p, 4, hipMemcpyDeviceToHost));
+    *loss = l / (float)n;
+    hipLaunchKernelGGL(k_zero, dim3((kParams + 255) / 256), dim3(256), 0, 0, m->G.f(), kParams);
+    MLP_TRY(hipGetLastError());
+    return NGAP_OK;
+  });
 }
 
 int ngap_mlp_train(ngap_mlp *m, const float *x, const float *t, uint64_t n, int32_t epochs,
                    uint32_t batch, float lr, uint64_t seed, float *final_loss) {
-  if (!m || !x || !t || !n || epochs < 1 || batch < 1)
-    return fail(NGAP_ERR_INVALID, "bad argument");
-  MLP_TRY(hipSetDevice(m->device));
-  // shuffled copy per epoch on the host; each mini-batch is uploaded once
-  std::vector<uint64_t> idx(n);
-  for (uint64_t i = 0; i < n; i++) idx[i] = i;
-  std::vector<float> bx((size_t)batch * kIn), bt(batch);
-  uint64_t z = seed;
-  double last = 0;
-  for (int e = 0; e < epochs; e++) {
-    for (uint64_t i = n - 1; i > 0; i--) {
-      z = z * 6364136223846793005ull + 1442695040888963407ull;
-      std::swap(idx[i], idx[(z >> 33) % (i + 1)]);
-    }
-    double lsum = 0;
-    uint64_t cnt = 0;
-    for (uint64_t b0 = 0; b0 < n; b0 += batch) {
-      const uint64_t nb = std::min<uint64_t>(batch, n - b0);
-      for (uint64_t k = 0; k < nb; k++) {
-        memcpy(&bx[k * kIn], x + idx[b0 + k] * kIn, kIn * 4);
-        bt[k] = t[idx[b0 + k]];
+  return ngapamd::c_guard([&]() -> int {
+    if (!m || !x || !t || !n || epochs < 1 || batch < 1)
+      return fail(NGAP_ERR_INVALID, "bad argument");
+    MLP_TRY(hipSetDevice(m->device));
+    // shuffled copy per epoch on the host; each mini-batch is uploaded once
+    std::vector<uint64_t> idx(n);
+    for (uint64_t i = 0; i < n; i++) idx[i] = i;
+    std::vector<float> bx((size_t)batch * kIn), bt(batch);
+    uint64_t z = seed;
+    double last = 0;
+    for (int e = 0; e < epochs; e++) {
+      for (uint64_t i = n - 1; i > 0; i--) {
+        z = z * 6364136223846793005ull + 1442695040888963407ull;
+        std::swap(idx[i], idx[(z >> 33) % (i + 1)]);
       }
-      int rc = load_batch(m, bx.data(), bt.data(), nb);
-      if (rc) return rc;
-      rc = grad_on_device(m, nb);
-      if (rc) return rc;
-      m->step++;
-      const float b1 = 0.9f, b2 = 0.999f;
-      const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
-      hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, 0, m->P.f(), m->G.f(),
-                         m->M.f(), m->V.f(), kParams, lr, b1, b2, 1e-8f, c1, c2);
-      MLP_TRY(hipGetLastError());
-      float l = 0;
-      MLP_TRY(hipMemcpy(&l, m->loss.p, 4, hipMemcpyDeviceToHost));
-      lsum += l;
-      cnt += nb;
+      double lsum = 0;
+      uint64_t cnt = 0;
+      for (uint64_t b0 = 0; b0 < n; b0 += batch) {
+        const uint64_t nb = std::min<uint64_t>(batch, n - b0);
+        for (uint64_t k = 0; k < nb; k++) {
+          memcpy(&bx[k * kIn], x + idx[b0 + k] * kIn, kIn * 4);
+          bt[k] = t[idx[b0 + k]];
+        }
+        int rc = load_batch(m, bx.data(), bt.data(), nb);
+        if (rc) return rc;
+        rc = grad_on_device(m, nb);
+        if (rc) return rc;
+        m->step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+        hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, 0, m->P.f(), m->G.f(),
+                           m->M.f(), m->V.f(), kParams, lr, b1, b2, 1e-8f, c1, c2);
+        MLP_TRY(hipGetLastError());
+        float l = 0;
+        MLP_TRY(hipMemcpy(&l, m->loss.p, 4, hipMemcpyDeviceToHost));
+        lsum += l;
+        cnt += nb;
+      }
+      last = lsum / (double)cnt;
     }
-    last = lsum / (double)cnt;
-  }
-  if (final_loss) *final_loss = (float)last;
-  return NGAP_OK;
+    if (final_loss) *final_loss = (float)last;
+    return NGAP_OK;
+  });
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 // with the engine (resident across runs) instead of per launch, and errors are
 // status codes instead of exit().
 #include "runtime.h"
+#include "c_guard.h"
 
 #include <algorithm>
 #include <array>
@@ -26,8 +27,10 @@ static uint64_t next_schedule_id() {
 }
 
 int ngap_fail(int code, const std::string &msg) {
-  ngap_set_last_error_internal(msg);
-  return code;
+  return ngapamd::c_guard([&]() -> int {
+    ngap_set_last_error_internal(msg);
+    return code;
+  });
 }
 
 namespace ngapamd {
@@ -658,33 +661,37 @@ void ngap_default_opts(ngap_run_opts *o) {
 }
 
 int ngap_create(int32_t device, ngap_engine **out) {
-  if (!out) return ngap_fail(NGAP_ERR_INVALID, "out is NULL");
-  *out = nullptr;
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
-    return ngap_fail(NGAP_ERR_DEVICE, "no HIP device available");
-  if (device < 0 || device >= count) return ngap_fail(NGAP_ERR_INVALID, "bad device ordinal");
-  std::unique_ptr<ngap_engine> e(new ngap_engine());
-  e->device = device;
-  HIP_TRY(hipSetDevice(device));
-  int rc = e->ctx.init();
-  if (rc) return rc;
-  hipDeviceProp_t prop;
-  HIP_TRY(hipGetDeviceProperties(&prop, device));
-  e->num_cus = prop.multiProcessorCount;
-  HIP_TRY(set_lds_limit(lds_bytes(kMaxRing)));
-  rc = noalloc_attach(device, &e->scratch);  // this device's scratch array (noalloc.cpp)
-  if (rc) return rc;
-  *out = e.release();
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!out) return ngap_fail(NGAP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      return ngap_fail(NGAP_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return ngap_fail(NGAP_ERR_INVALID, "bad device ordinal");
+    std::unique_ptr<ngap_engine> e(new ngap_engine());
+    e->device = device;
+    HIP_TRY(hipSetDevice(device));
+    int rc = e->ctx.init();
+    if (rc) return rc;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    e->num_cus = prop.multiProcessorCount;
+    HIP_TRY(set_lds_limit(lds_bytes(kMaxRing)));
+    rc = noalloc_attach(device, &e->scratch);  // this device's scratch array (noalloc.cpp)
+    if (rc) return rc;
+    *out = e.release();
+    return NGAP_OK;
+  });
 }
 
 int ngap_destroy(ngap_engine *eng) {
-  if (!eng) return NGAP_OK;
-  (void)hipSetDevice(eng->device);
-  if (eng->ctx.stream) (void)hipStreamSynchronize(eng->ctx.stream);
-  delete eng;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng) return NGAP_OK;
+    (void)hipSetDevice(eng->device);
+    if (eng->ctx.stream) (void)hipStreamSynchronize(eng->ctx.stream);
+    delete eng;
+    return NGAP_OK;
+  });
 }
 
 namespace {
@@ -744,422 +751,444 @@ int pack_image(ngap_engine *eng) {
 
 int ngap_load_groups_ex(ngap_engine *eng, const ngap_group_desc *groups, int32_t num_groups,
                         int32_t max_depth, int32_t build) {
-  if (!eng || !groups) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (build != NGAP_BUILD_DEVICE && build != NGAP_BUILD_HOST)
-    return ngap_fail(NGAP_ERR_INVALID, "build must be NGAP_BUILD_DEVICE or NGAP_BUILD_HOST");
-  HIP_TRY(hipSetDevice(eng->device));
-  const auto t0 = std::chrono::steady_clock::now();
-  std::vector<HostGroup> hg(num_groups > 0 ? num_groups : 0);
-  for (int i = 0; i < num_groups; i++) {
-    hg[i].num_states = groups[i].num_states;
-    hg[i].row = groups[i].row_offsets;
-    hg[i].col = groups[i].col_indices;
-    hg[i].sets = groups[i].symbol_sets;
-    hg[i].attrs = groups[i].node_attrs;
-    hg[i].num_aas = groups[i].num_always_active;
-    hg[i].aas = groups[i].always_active;
-    hg[i].num_sod = groups[i].num_start;
-    hg[i].sod = groups[i].start_states;
-  }
-  std::string err;
-  eng->loaded = false;
-  eng->generation++;
-  eng->ctx.have_run = false;
-  eng->ctx.drop_graph();
-  const int depth = max_depth < 1 ? 1 : max_depth;
-  if (build == NGAP_BUILD_DEVICE) {
-    const int rc = build_image_device(eng, hg.data(), num_groups, depth, eng->ctx.stream,
-                                      eng->img, eng->img_sizes, err);
-    if (rc) return err.empty() ? rc : ngap_fail(rc, err);
-  } else {
-    if (!build_image(hg.data(), num_groups, depth, eng->img, err))
-      return ngap_fail(NGAP_ERR_INVALID, err);
-    Image &im = eng->img;
-    HIP_TRY(eng->d_meta.upload(im.meta));
-    HIP_TRY(eng->d_succ4.upload(im.succ4));
-    HIP_TRY(eng->d_cls.upload(im.cls));
-    HIP_TRY(eng->d_sod.upload(im.sod));
-    HIP_TRY(eng->d_mask1.upload(im.mask1));
-    HIP_TRY(eng->d_xr1.upload(im.xr1));
-    HIP_TRY(eng->d_xe1.upload(im.xe1));
-    HIP_TRY(eng->d_xmask1.upload(im.xmask1));
-    HIP_TRY(eng->d_xr2.upload(im.xr2));
-    HIP_TRY(eng->d_xe2.upload(im.xe2));
-    HIP_TRY(eng->d_xsod.upload(im.xsod));
-    HIP_TRY(eng->d_xs3.upload(im.xs3));
-    HIP_TRY(eng->d_xp3.upload(im.xp3));
-    HIP_TRY(eng->d_x2rep.upload(im.x2rep));
-    for (int d = 0; d < kMaxMemoDepth; d++) {
-      HIP_TRY(eng->d_beg[d].upload(im.memo[d].beg));
-      HIP_TRY(eng->d_nrep[d].upload(im.memo[d].nrep));
-      HIP_TRY(eng->d_list[d].upload(im.memo[d].list));
-      HIP_TRY(eng->d_sel[d].upload(im.memo[d].sel));
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !groups) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (build != NGAP_BUILD_DEVICE && build != NGAP_BUILD_HOST)
+      return ngap_fail(NGAP_ERR_INVALID, "build must be NGAP_BUILD_DEVICE or NGAP_BUILD_HOST");
+    HIP_TRY(hipSetDevice(eng->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<HostGroup> hg(num_groups > 0 ? num_groups : 0);
+    for (int i = 0; i < num_groups; i++) {
+      hg[i].num_states = groups[i].num_states;
+      hg[i].row = groups[i].row_offsets;
+      hg[i].col = groups[i].col_indices;
+      hg[i].sets = groups[i].symbol_sets;
+      hg[i].attrs = groups[i].node_attrs;
+      hg[i].num_aas = groups[i].num_always_active;
+      hg[i].aas = groups[i].always_active;
+      hg[i].num_sod = groups[i].num_start;
+      hg[i].sod = groups[i].start_states;
     }
-    eng->img_sizes = sizes_of(im);
-    // the host keeps what the runtime reads (group metadata, root masks)
-    Image kept;
-    kept.num_groups = im.num_groups;
-    kept.memo_depth = im.memo_depth;
-    kept.meta.swap(im.meta);
-    kept.mask1.swap(im.mask1);
-    kept.n_xsod = im.n_xsod;
-    kept.total_states = im.total_states;
-    im = std::move(kept);
-  }
-  if (const int rc = pack_image(eng)) return rc;
-  eng->built_on_device = build == NGAP_BUILD_DEVICE;
-  eng->build_ms =
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  eng->loaded = true;
-  eng->generation++;
-  return NGAP_OK;
+    std::string err;
+    eng->loaded = false;
+    eng->generation++;
+    eng->ctx.have_run = false;
+    eng->ctx.drop_graph();
+    const int depth = max_depth < 1 ? 1 : max_depth;
+    if (build == NGAP_BUILD_DEVICE) {
+      const int rc = build_image_device(eng, hg.data(), num_groups, depth, eng->ctx.stream,
+                                        eng->img, eng->img_sizes, err);
+      if (rc) return err.empty() ? rc : ngap_fail(rc, err);
+    } else {
+      if (!build_image(hg.data(), num_groups, depth, eng->img, err))
+        return ngap_fail(NGAP_ERR_INVALID, err);
+      Image &im = eng->img;
+      HIP_TRY(eng->d_meta.upload(im.meta));
+      HIP_TRY(eng->d_succ4.upload(im.succ4));
+      HIP_TRY(eng->d_cls.upload(im.cls));
+      HIP_TRY(eng->d_sod.upload(im.sod));
+      HIP_TRY(eng->d_mask1.upload(im.mask1));
+      HIP_TRY(eng->d_xr1.upload(im.xr1));
+      HIP_TRY(eng->d_xe1.upload(im.xe1));
+      HIP_TRY(eng->d_xmask1.upload(im.xmask1));
+      HIP_TRY(eng->d_xr2.upload(im.xr2));
+      HIP_TRY(eng->d_xe2.upload(im.xe2));
+      HIP_TRY(eng->d_xsod.upload(im.xsod));
+      HIP_TRY(eng->d_xs3.upload(im.xs3));
+      HIP_TRY(eng->d_xp3.upload(im.xp3));
+      HIP_TRY(eng->d_x2rep.upload(im.x2rep));
+      for (int d = 0; d < kMaxMemoDepth; d++) {
+        HIP_TRY(eng->d_beg[d].upload(im.memo[d].beg));
+        HIP_TRY(eng->d_nrep[d].upload(im.memo[d].nrep));
+        HIP_TRY(eng->d_list[d].upload(im.memo[d].list));
+        HIP_TRY(eng->d_sel[d].upload(im.memo[d].sel));
+      }
+      eng->img_sizes = sizes_of(im);
+      // the host keeps what the runtime reads (group metadata, root masks)
+      Image kept;
+      kept.num_groups = im.num_groups;
+      kept.memo_depth = im.memo_depth;
+      kept.meta.swap(im.meta);
+      kept.mask1.swap(im.mask1);
+      kept.n_xsod = im.n_xsod;
+      kept.total_states = im.total_states;
+      im = std::move(kept);
+    }
+    if (const int rc = pack_image(eng)) return rc;
+    eng->built_on_device = build == NGAP_BUILD_DEVICE;
+    eng->build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    eng->loaded = true;
+    eng->generation++;
+    return NGAP_OK;
+  });
 }
 
 int ngap_load_groups(ngap_engine *eng, const ngap_group_desc *groups, int32_t num_groups,
                      int32_t max_depth) {
-  return ngap_load_groups_ex(eng, groups, num_groups, max_depth, NGAP_BUILD_DEVICE);
+  return ngapamd::c_guard([&]() -> int {
+    return ngap_load_groups_ex(eng, groups, num_groups, max_depth, NGAP_BUILD_DEVICE);
+  });
 }
 
 int ngap_image_info(ngap_engine *eng, ngap_image_summary *info) {
-  if (!eng || !info) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
-  memset(info, 0, sizeof *info);
-  info->built_on_device = eng->built_on_device ? 1 : 0;
-  info->memo_depth = eng->img.memo_depth;
-  info->num_groups = eng->img.num_groups;
-  info->num_states = eng->img.total_states;
-  info->bytes = eng->img_sizes.total();
-  info->build_ms = eng->build_ms;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !info) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
+    memset(info, 0, sizeof *info);
+    info->built_on_device = eng->built_on_device ? 1 : 0;
+    info->memo_depth = eng->img.memo_depth;
+    info->num_groups = eng->img.num_groups;
+    info->num_states = eng->img.total_states;
+    info->bytes = eng->img_sizes.total();
+    info->build_ms = eng->build_ms;
+    return NGAP_OK;
+  });
 }
 
 int ngap_image_array(ngap_engine *eng, const char *name, void *out, uint64_t cap,
                      uint64_t *n_bytes) {
-  if (!eng || !name || !n_bytes) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
-  const ImageSizes &z = eng->img_sizes;
-  struct Entry {
-    const char *name;
-    const DevBuf *buf;
-    uint64_t bytes;
-  };
-  const Entry table[] = {
-      {"meta", &eng->d_meta, z.meta},       {"succ4", &eng->d_succ4, z.succ4},
-      {"cls", &eng->d_cls, z.cls},          {"sod", &eng->d_sod, z.sod},
-      {"beg1", &eng->d_beg[0], z.beg[0]},   {"beg2", &eng->d_beg[1], z.beg[1]},
-      {"beg3", &eng->d_beg[2], z.beg[2]},   {"nrep1", &eng->d_nrep[0], z.nrep[0]},
-      {"nrep2", &eng->d_nrep[1], z.nrep[1]}, {"nrep3", &eng->d_nrep[2], z.nrep[2]},
-      {"list1", &eng->d_list[0], z.list[0]}, {"list2", &eng->d_list[1], z.list[1]},
-      {"list3", &eng->d_list[2], z.list[2]}, {"sel2", &eng->d_sel[1], z.sel[1]},
-      {"sel3", &eng->d_sel[2], z.sel[2]},   {"mask1", &eng->d_mask1, z.mask1},
-      {"xr1", &eng->d_xr1, z.xr1},          {"xe1", &eng->d_xe1, z.xe1},
-      {"xmask1", &eng->d_xmask1, z.xmask1}, {"xr2", &eng->d_xr2, z.xr2},
-      {"xe2", &eng->d_xe2, z.xe2},          {"xsod", &eng->d_xsod, z.xsod},
-      {"xs3", &eng->d_xs3, z.xs3},          {"xp3", &eng->d_xp3, z.xp3},
-      {"x2rep", &eng->d_x2rep, z.x2rep}};
-  for (const Entry &e : table) {
-    if (strcmp(e.name, name) != 0) continue;
-    *n_bytes = e.bytes;
-    if (out && e.bytes) {
-      HIP_TRY(hipSetDevice(eng->device));
-      HIP_TRY(hipMemcpy(out, e.buf->p, std::min<uint64_t>(cap, e.bytes), hipMemcpyDeviceToHost));
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !name || !n_bytes) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
+    const ImageSizes &z = eng->img_sizes;
+    struct Entry {
+      const char *name;
+      const DevBuf *buf;
+      uint64_t bytes;
+    };
+    const Entry table[] = {
+        {"meta", &eng->d_meta, z.meta},       {"succ4", &eng->d_succ4, z.succ4},
+        {"cls", &eng->d_cls, z.cls},          {"sod", &eng->d_sod, z.sod},
+        {"beg1", &eng->d_beg[0], z.beg[0]},   {"beg2", &eng->d_beg[1], z.beg[1]},
+        {"beg3", &eng->d_beg[2], z.beg[2]},   {"nrep1", &eng->d_nrep[0], z.nrep[0]},
+        {"nrep2", &eng->d_nrep[1], z.nrep[1]}, {"nrep3", &eng->d_nrep[2], z.nrep[2]},
+        {"list1", &eng->d_list[0], z.list[0]}, {"list2", &eng->d_list[1], z.list[1]},
+        {"list3", &eng->d_list[2], z.list[2]}, {"sel2", &eng->d_sel[1], z.sel[1]},
+        {"sel3", &eng->d_sel[2], z.sel[2]},   {"mask1", &eng->d_mask1, z.mask1},
+        {"xr1", &eng->d_xr1, z.xr1},          {"xe1", &eng->d_xe1, z.xe1},
+        {"xmask1", &eng->d_xmask1, z.xmask1}, {"xr2", &eng->d_xr2, z.xr2},
+        {"xe2", &eng->d_xe2, z.xe2},          {"xsod", &eng->d_xsod, z.xsod},
+        {"xs3", &eng->d_xs3, z.xs3},          {"xp3", &eng->d_xp3, z.xp3},
+        {"x2rep", &eng->d_x2rep, z.x2rep}};
+    for (const Entry &e : table) {
+      if (strcmp(e.name, name) != 0) continue;
+      *n_bytes = e.bytes;
+      if (out && e.bytes) {
+        HIP_TRY(hipSetDevice(eng->device));
+        HIP_TRY(hipMemcpy(out, e.buf->p, std::min<uint64_t>(cap, e.bytes), hipMemcpyDeviceToHost));
+      }
+      return NGAP_OK;
     }
-    return NGAP_OK;
-  }
-  return ngap_fail(NGAP_ERR_INVALID, std::string("unknown image array ") + name);
+    return ngap_fail(NGAP_ERR_INVALID, std::string("unknown image array ") + name);
+  });
 }
 
 int ngap_set_input(ngap_engine *eng, const uint8_t *data, uint64_t chunk_len,
                    uint32_t num_seg) {
-  if (!eng || (!data && chunk_len * num_seg)) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (chunk_len == 0 || num_seg == 0) return ngap_fail(NGAP_ERR_INVALID, "empty input");
-  // offsets are 32-bit; keep headroom for the position cursors
-  if (chunk_len * num_seg >= (1ull << 32) - (1ull << 20))
-    return ngap_fail(NGAP_ERR_INVALID,
-                     "input offsets are 32-bit: total input must be < 4 GiB - 1 MiB");
-  HIP_TRY(hipSetDevice(eng->device));
-  size_t n = (size_t)chunk_len * num_seg;
-  HIP_TRY(eng->d_input.reserve(n + 16));
-  // The input goes through two pinned blocks: host threads copy a block in
-  // (counting its bytes for the static per-group schedule on the way) while
-  // the other block's copy to the device runs.
-  InputStage &stg = eng->stage;
-  HIP_TRY(stg.init());
-  HostPool &pool = *stg.pool;
-  std::vector<std::array<uint64_t, 256>> hist(pool.width());
-  for (auto &h : hist) h.fill(0);
-  for (size_t off = 0, k = 0; off < n; off += InputStage::kBlock, k++) {
-    const size_t len = std::min(InputStage::kBlock, n - off);
-    uint8_t *st = stg.buf[k & 1];
-    if (k >= 2) HIP_TRY(hipEventSynchronize(stg.done[k & 1]));  // its last copy is done
-    const uint8_t *src = data + off;
-    pool.run_parts(len, [&](size_t part, size_t a, size_t b) {
-      std::copy(src + a, src + b, st + a);
-      // four partial counts: no store-to-load dependence between neighbours
-      uint32_t h[4][256] = {{0}};
-      size_t i = a;
-      for (; i + 4 <= b; i += 4) {
-        h[0][src[i]]++;
-        h[1][src[i + 1]]++;
-        h[2][src[i + 2]]++;
-        h[3][src[i + 3]]++;
-      }
-      for (; i < b; i++) h[0][src[i]]++;
-      for (int x = 0; x < 256; x++) hist[part][x] += (uint64_t)h[0][x] + h[1][x] + h[2][x] + h[3][x];
-    });
-    HIP_TRY(hipMemcpyAsync(eng->d_input.as<uint8_t>() + off, st, len, hipMemcpyHostToDevice,
-                           eng->ctx.stream));
-    HIP_TRY(hipEventRecord(stg.done[k & 1], eng->ctx.stream));
-  }
-  HIP_TRY(hipStreamSynchronize(eng->ctx.stream));
-  for (int b = 0; b < 256; b++) {
-    uint64_t c = 0;
-    for (auto &h : hist) c += h[b];
-    eng->byte_freq[b] = (double)c / (double)n;
-  }
-  eng->chunk_len = chunk_len;
-  eng->num_seg = num_seg;
-  eng->have_input = true;
-  eng->generation++;
-  eng->ctx.have_run = false;
-  // a captured graph stays: its launch parameters (input address, chunk
-  // sizes) are compared with the next run's before it is replayed
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || (!data && chunk_len * num_seg)) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (chunk_len == 0 || num_seg == 0) return ngap_fail(NGAP_ERR_INVALID, "empty input");
+    // offsets are 32-bit; keep headroom for the position cursors
+    if (chunk_len * num_seg >= (1ull << 32) - (1ull << 20))
+      return ngap_fail(NGAP_ERR_INVALID,
+                       "input offsets are 32-bit: total input must be < 4 GiB - 1 MiB");
+    HIP_TRY(hipSetDevice(eng->device));
+    size_t n = (size_t)chunk_len * num_seg;
+    HIP_TRY(eng->d_input.reserve(n + 16));
+    // The input goes through two pinned blocks: host threads copy a block in
+    // (counting its bytes for the static per-group schedule on the way) while
+    // the other block's copy to the device runs.
+    InputStage &stg = eng->stage;
+    HIP_TRY(stg.init());
+    HostPool &pool = *stg.pool;
+    std::vector<std::array<uint64_t, 256>> hist(pool.width());
+    for (auto &h : hist) h.fill(0);
+    for (size_t off = 0, k = 0; off < n; off += InputStage::kBlock, k++) {
+      const size_t len = std::min(InputStage::kBlock, n - off);
+      uint8_t *st = stg.buf[k & 1];
+      if (k >= 2) HIP_TRY(hipEventSynchronize(stg.done[k & 1]));  // its last copy is done
+      const uint8_t *src = data + off;
+      pool.run_parts(len, [&](size_t part, size_t a, size_t b) {
+        std::copy(src + a, src + b, st + a);
+        // four partial counts: no store-to-load dependence between neighbours
+        uint32_t h[4][256] = {{0}};
+        size_t i = a;
+        for (; i + 4 <= b; i += 4) {
+          h[0][src[i]]++;
+          h[1][src[i + 1]]++;
+          h[2][src[i + 2]]++;
+          h[3][src[i + 3]]++;
+        }
+        for (; i < b; i++) h[0][src[i]]++;
+        for (int x = 0; x < 256; x++) hist[part][x] += (uint64_t)h[0][x] + h[1][x] + h[2][x] + h[3][x];
+      });
+      HIP_TRY(hipMemcpyAsync(eng->d_input.as<uint8_t>() + off, st, len, hipMemcpyHostToDevice,
+                             eng->ctx.stream));
+      HIP_TRY(hipEventRecord(stg.done[k & 1], eng->ctx.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(eng->ctx.stream));
+    for (int b = 0; b < 256; b++) {
+      uint64_t c = 0;
+      for (auto &h : hist) c += h[b];
+      eng->byte_freq[b] = (double)c / (double)n;
+    }
+    eng->chunk_len = chunk_len;
+    eng->num_seg = num_seg;
+    eng->have_input = true;
+    eng->generation++;
+    eng->ctx.have_run = false;
+    // a captured graph stays: its launch parameters (input address, chunk
+    // sizes) are compared with the next run's before it is replayed
+    return NGAP_OK;
+  });
 }
 
 int ngap_run(ngap_engine *eng, const ngap_run_opts *opts, ngap_run_stats *st) {
-  if (!eng || !opts) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (st) memset(st, 0, sizeof *st);
-  ngap_run_opts o = *opts;  // a copy: the default grid is resolved below
-  if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
-  if (!eng->have_input) return ngap_fail(NGAP_ERR_STATE, "no input set");
-  HIP_TRY(hipSetDevice(eng->device));
-  Mapping mp;
-  std::string err;
-  if (!map_algorithm(o.algorithm, o.precompute_depth, eng->img.memo_depth, mp, err))
-    return ngap_fail(NGAP_ERR_INVALID, err);
-  if (!ring_of(o))
-    return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
-  const uint64_t units = (uint64_t)eng->img.num_groups * eng->num_seg;
-  if (units >= (1u << 31)) return ngap_fail(NGAP_ERR_INVALID, "too many (group, chunk) units");
-  RunCtx &ctx = eng->ctx;
-  const uint64_t cap = o.result_capacity ? o.result_capacity : 54619400ull;
-  if (!o.report_off) HIP_TRY(ctx.reports.reserve(cap * 8));
-  ctx.report_cap = o.report_off ? 0 : cap;
-  KParams P = base_params(eng, mp, o);
-  P.input = eng->d_input.as<uint8_t>();
-  P.chunk_len = (uint32_t)eng->chunk_len;
-  P.chunk_magic = ~0ull / P.chunk_len + 1;  // wraps to 0 for 1-byte chunks, as the test needs
-  P.num_seg = eng->num_seg;
-  if (st) st->num_units = (uint32_t)units;
-  if (o.schedule == NGAP_SCHED_PLAIN || o.algorithm == NGAP_ALGO_BLOCKING)
-    return run_plain(eng, ctx, P, o, st);
-  // the schedule: host work items, all-group windows or per-group windows,
-  // rebuilt only when its inputs change
-  ngap_engine::SchedCache &sc = eng->sched;
-  // all-group windows wherever the cross-group index exists (depth >= 2),
-  // one group included: a 3-gram slot per position replaces the group's
-  // rank-table lookups (regex20k: 0.185 ms against 0.217)
-  const int kind = o.tasks ? 0 : (mp.memo_depth >= 2 && o.cross_groups != 0) ? 1 : 2;
-  // the default grid, sized to the job
-  if (o.persistent_blocks == 0) {
-    uint32_t full = 0;
-    int rc = persistent_blocks(eng, 0, &full, P.ring_cap, engine_kind(o));
-    if (rc) return rc;
-    uint64_t work = 0;
-    if (kind == 0) {
-      for (uint32_t i = 0; i < o.num_tasks; i++)
-        work += o.tasks[i].end > o.tasks[i].begin ? o.tasks[i].end - o.tasks[i].begin : 0;
-    } else {
-      work = (uint64_t)eng->num_seg * eng->chunk_len * (kind == 2 ? eng->img.num_groups : 1);
-    }
-    o.persistent_blocks = job_blocks(full, P.waves_per_block, work);
-  }
-  uint32_t waves = 0;
-  if (kind != 0) {
-    uint32_t blocks = 0;
-    int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap, engine_kind(o));
-    if (rc) return rc;
-    waves = blocks * P.waves_per_block;
-  }
-  bool hit = sc.id && sc.generation == eng->generation && sc.kind == kind &&
-             sc.is_dup == (o.is_dup != 0);
-  if (hit && kind == 0)
-    hit = sc.memo_depth == mp.memo_depth && sc.host.size() == o.num_tasks &&
-          (o.num_tasks == 0 || (o.tasks_id && o.tasks_id == sc.id) ||
-           memcmp(sc.host.data(), o.tasks, o.num_tasks * sizeof(ngap_task)) == 0);
-  else if (hit)
-    hit = sc.window == o.window && sc.waves == waves;
-  if (!hit) {
-    sc.id = 0;
-    if (kind == 0) {
-      int rc = host_tasks(eng, o, mp.memo_depth, sc.tasks);
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !opts) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (st) memset(st, 0, sizeof *st);
+    ngap_run_opts o = *opts;  // a copy: the default grid is resolved below
+    if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
+    if (!eng->have_input) return ngap_fail(NGAP_ERR_STATE, "no input set");
+    HIP_TRY(hipSetDevice(eng->device));
+    Mapping mp;
+    std::string err;
+    if (!map_algorithm(o.algorithm, o.precompute_depth, eng->img.memo_depth, mp, err))
+      return ngap_fail(NGAP_ERR_INVALID, err);
+    if (!ring_of(o))
+      return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
+    const uint64_t units = (uint64_t)eng->img.num_groups * eng->num_seg;
+    if (units >= (1u << 31)) return ngap_fail(NGAP_ERR_INVALID, "too many (group, chunk) units");
+    RunCtx &ctx = eng->ctx;
+    const uint64_t cap = o.result_capacity ? o.result_capacity : 54619400ull;
+    if (!o.report_off) HIP_TRY(ctx.reports.reserve(cap * 8));
+    ctx.report_cap = o.report_off ? 0 : cap;
+    KParams P = base_params(eng, mp, o);
+    P.input = eng->d_input.as<uint8_t>();
+    P.chunk_len = (uint32_t)eng->chunk_len;
+    P.chunk_magic = ~0ull / P.chunk_len + 1;  // wraps to 0 for 1-byte chunks, as the test needs
+    P.num_seg = eng->num_seg;
+    if (st) st->num_units = (uint32_t)units;
+    if (o.schedule == NGAP_SCHED_PLAIN || o.algorithm == NGAP_ALGO_BLOCKING)
+      return run_plain(eng, ctx, P, o, st);
+    // the schedule: host work items, all-group windows or per-group windows,
+    // rebuilt only when its inputs change
+    ngap_engine::SchedCache &sc = eng->sched;
+    // all-group windows wherever the cross-group index exists (depth >= 2),
+    // one group included: a 3-gram slot per position replaces the group's
+    // rank-table lookups (regex20k: 0.185 ms against 0.217)
+    const int kind = o.tasks ? 0 : (mp.memo_depth >= 2 && o.cross_groups != 0) ? 1 : 2;
+    // the default grid, sized to the job
+    if (o.persistent_blocks == 0) {
+      uint32_t full = 0;
+      int rc = persistent_blocks(eng, 0, &full, P.ring_cap, engine_kind(o));
       if (rc) return rc;
-      sc.host.assign(o.tasks, o.tasks + o.num_tasks);
-    } else if (kind == 1) {
-      sc.tasks = cross_schedule(eng, o.window, o.is_dup != 0, waves);
-    } else {
-      sc.tasks = static_schedule(eng, o.window, o.is_dup != 0, waves);
+      uint64_t work = 0;
+      if (kind == 0) {
+        for (uint32_t i = 0; i < o.num_tasks; i++)
+          work += o.tasks[i].end > o.tasks[i].begin ? o.tasks[i].end - o.tasks[i].begin : 0;
+      } else {
+        work = (uint64_t)eng->num_seg * eng->chunk_len * (kind == 2 ? eng->img.num_groups : 1);
+      }
+      o.persistent_blocks = job_blocks(full, P.waves_per_block, work);
     }
-    sc.generation = eng->generation;
-    sc.kind = kind;
-    sc.is_dup = o.is_dup != 0;
-    sc.memo_depth = mp.memo_depth;
-    sc.window = o.window;
-    sc.waves = waves;
-    sc.id = next_schedule_id();
-  }
-  const int rc = run_tasks(eng, ctx, P, sc.tasks, sc.id, o, st);
-  if (st) st->tasks_id = sc.id;
-  return rc;
+    uint32_t waves = 0;
+    if (kind != 0) {
+      uint32_t blocks = 0;
+      int rc = persistent_blocks(eng, o.persistent_blocks, &blocks, P.ring_cap, engine_kind(o));
+      if (rc) return rc;
+      waves = blocks * P.waves_per_block;
+    }
+    bool hit = sc.id && sc.generation == eng->generation && sc.kind == kind &&
+               sc.is_dup == (o.is_dup != 0);
+    if (hit && kind == 0)
+      hit = sc.memo_depth == mp.memo_depth && sc.host.size() == o.num_tasks &&
+            (o.num_tasks == 0 || (o.tasks_id && o.tasks_id == sc.id) ||
+             memcmp(sc.host.data(), o.tasks, o.num_tasks * sizeof(ngap_task)) == 0);
+    else if (hit)
+      hit = sc.window == o.window && sc.waves == waves;
+    if (!hit) {
+      sc.id = 0;
+      if (kind == 0) {
+        int rc = host_tasks(eng, o, mp.memo_depth, sc.tasks);
+        if (rc) return rc;
+        sc.host.assign(o.tasks, o.tasks + o.num_tasks);
+      } else if (kind == 1) {
+        sc.tasks = cross_schedule(eng, o.window, o.is_dup != 0, waves);
+      } else {
+        sc.tasks = static_schedule(eng, o.window, o.is_dup != 0, waves);
+      }
+      sc.generation = eng->generation;
+      sc.kind = kind;
+      sc.is_dup = o.is_dup != 0;
+      sc.memo_depth = mp.memo_depth;
+      sc.window = o.window;
+      sc.waves = waves;
+      sc.id = next_schedule_id();
+    }
+    const int rc = run_tasks(eng, ctx, P, sc.tasks, sc.id, o, st);
+    if (st) st->tasks_id = sc.id;
+    return rc;
+  });
 }
 
 int ngap_autotune(ngap_engine *eng, const ngap_run_opts *base, int32_t max_runs,
                   ngap_run_opts *best, double *best_ms) {
-  if (!eng || !base || !best) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (!eng->loaded || !eng->have_input) return ngap_fail(NGAP_ERR_STATE, "no groups or input");
-  HIP_TRY(hipSetDevice(eng->device));
-  ngap_run_opts o = *base;
-  o.schedule = NGAP_SCHED_BATCHED;
-  o.fallback = 1;
-  o.tasks = nullptr;
-  o.num_tasks = 0;
-  const uint32_t L = (uint32_t)eng->chunk_len;
-  int runs = 0;
-  double best_t = 1e300;
-  ngap_run_opts cur = o;
-  // time one setting: the faster of two runs (kernel plus fallback time)
-  auto timeit = [&](const ngap_run_opts &c, double &t) -> int {
-    t = 1e300;
-    for (int r = 0; r < 2; r++) {
-      ngap_run_stats st;
-      int rc = ngap_run(eng, &c, &st);
-      if (rc) return rc;
-      t = std::min(t, st.kernel_ms + st.fallback_ms);
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !base || !best) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (!eng->loaded || !eng->have_input) return ngap_fail(NGAP_ERR_STATE, "no groups or input");
+    HIP_TRY(hipSetDevice(eng->device));
+    ngap_run_opts o = *base;
+    o.schedule = NGAP_SCHED_BATCHED;
+    o.fallback = 1;
+    o.tasks = nullptr;
+    o.num_tasks = 0;
+    const uint32_t L = (uint32_t)eng->chunk_len;
+    int runs = 0;
+    double best_t = 1e300;
+    ngap_run_opts cur = o;
+    // time one setting: the faster of two runs (kernel plus fallback time)
+    auto timeit = [&](const ngap_run_opts &c, double &t) -> int {
+      t = 1e300;
+      for (int r = 0; r < 2; r++) {
+        ngap_run_stats st;
+        int rc = ngap_run(eng, &c, &st);
+        if (rc) return rc;
+        t = std::min(t, st.kernel_ms + st.fallback_ms);
+      }
+      runs++;
+      return NGAP_OK;
+    };
+    int rc = timeit(cur, best_t);
+    if (rc) return rc;
+    const std::vector<uint32_t> windows = {256, 512, 1024, 2048, 4096, 8192, 16384};
+    const std::vector<uint32_t> rings = {128, 512, 1024};
+    const std::vector<uint32_t> spills = {8192, 16384, 262144};
+    // the reference's tuning values of --active-threshold (ngap.cu:1999)
+    const std::vector<uint32_t> thresholds = {0, 8, 16, 24, 32};
+    for (int axis = 0; axis < 6; axis++) {
+      // the grid axis scales the full grid of the ring capacity picked so far
+      uint32_t full = 0;
+      if (axis == 2) {
+        ngap_run_opts f = cur;
+        f.persistent_blocks = 0;
+        int rc2 = persistent_blocks(eng, 0, &full, ring_of(f), engine_kind(f));
+        if (rc2) return rc2;
+      }
+      const std::vector<uint32_t> grids = {full / 4, full / 2, (full * 3) / 4};
+      const std::vector<uint32_t> passes = {cur.root_pass ? 0u : 1u};
+      const std::vector<uint32_t> &vals = axis == 0   ? windows
+                                          : axis == 1 ? rings
+                                          : axis == 2 ? grids
+                                          : axis == 3 ? spills
+                                          : axis == 4 ? passes
+                                                      : thresholds;
+      ngap_run_opts pick = cur;
+      for (uint32_t v : vals) {
+        if (runs >= max_runs) break;
+        ngap_run_opts c = cur;
+        if (axis == 0) {
+          if (v > L) continue;
+          c.window = v;
+        } else if (axis == 1) {
+          c.ring_capacity = v;
+        } else if (axis == 2) {
+          if (v == 0) continue;
+          c.persistent_blocks = v;
+        } else if (axis == 3) {
+          c.spill_capacity = v;
+        } else if (axis == 4) {
+          c.root_pass = (int32_t)v;
+        } else {
+          if ((int32_t)v == cur.active_threshold) continue;
+          c.active_threshold = (int32_t)v;
+        }
+        double t = 0;
+        rc = timeit(c, t);
+        if (rc) return rc;
+        if (t < best_t) {
+          best_t = t;
+          pick = c;
+        }
+      }
+      cur = pick;
     }
-    runs++;
+    *best = cur;
+    if (best_ms) *best_ms = best_t;
     return NGAP_OK;
-  };
-  int rc = timeit(cur, best_t);
-  if (rc) return rc;
-  const std::vector<uint32_t> windows = {256, 512, 1024, 2048, 4096, 8192, 16384};
-  const std::vector<uint32_t> rings = {128, 512, 1024};
-  const std::vector<uint32_t> spills = {8192, 16384, 262144};
-  // the reference's tuning values of --active-threshold (ngap.cu:1999)
-  const std::vector<uint32_t> thresholds = {0, 8, 16, 24, 32};
-  for (int axis = 0; axis < 6; axis++) {
-    // the grid axis scales the full grid of the ring capacity picked so far
-    uint32_t full = 0;
-    if (axis == 2) {
-      ngap_run_opts f = cur;
-      f.persistent_blocks = 0;
-      int rc2 = persistent_blocks(eng, 0, &full, ring_of(f), engine_kind(f));
-      if (rc2) return rc2;
-    }
-    const std::vector<uint32_t> grids = {full / 4, full / 2, (full * 3) / 4};
-    const std::vector<uint32_t> passes = {cur.root_pass ? 0u : 1u};
-    const std::vector<uint32_t> &vals = axis == 0   ? windows
-                                        : axis == 1 ? rings
-                                        : axis == 2 ? grids
-                                        : axis == 3 ? spills
-                                        : axis == 4 ? passes
-                                                    : thresholds;
-    ngap_run_opts pick = cur;
-    for (uint32_t v : vals) {
-      if (runs >= max_runs) break;
-      ngap_run_opts c = cur;
-      if (axis == 0) {
-        if (v > L) continue;
-        c.window = v;
-      } else if (axis == 1) {
-        c.ring_capacity = v;
-      } else if (axis == 2) {
-        if (v == 0) continue;
-        c.persistent_blocks = v;
-      } else if (axis == 3) {
-        c.spill_capacity = v;
-      } else if (axis == 4) {
-        c.root_pass = (int32_t)v;
-      } else {
-        if ((int32_t)v == cur.active_threshold) continue;
-        c.active_threshold = (int32_t)v;
-      }
-      double t = 0;
-      rc = timeit(c, t);
-      if (rc) return rc;
-      if (t < best_t) {
-        best_t = t;
-        pick = c;
-      }
-    }
-    cur = pick;
-  }
-  *best = cur;
-  if (best_ms) *best_ms = best_t;
-  return NGAP_OK;
+  });
 }
 
 int ngap_batched_grid(ngap_engine *eng, uint32_t ring_capacity, uint32_t *blocks,
                       uint32_t *waves_per_block) {
-  if (!eng || !blocks || !waves_per_block) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  ngap_run_opts o;
-  ngap_default_opts(&o);
-  o.ring_capacity = ring_capacity;
-  const uint32_t ring = ring_of(o);
-  if (!ring) return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
-  HIP_TRY(hipSetDevice(eng->device));
-  int rc = persistent_blocks(eng, 0, blocks, ring, engine_kind(o));
-  if (rc) return rc;
-  *waves_per_block = kBlockThreads / kWave;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !blocks || !waves_per_block) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    ngap_run_opts o;
+    ngap_default_opts(&o);
+    o.ring_capacity = ring_capacity;
+    const uint32_t ring = ring_of(o);
+    if (!ring) return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
+    HIP_TRY(hipSetDevice(eng->device));
+    int rc = persistent_blocks(eng, 0, blocks, ring, engine_kind(o));
+    if (rc) return rc;
+    *waves_per_block = kBlockThreads / kWave;
+    return NGAP_OK;
+  });
 }
 
 int ngap_get_reports(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out) {
-  if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  RunCtx &ctx = eng->ctx;
-  if (!ctx.have_run) return ngap_fail(NGAP_ERR_STATE, "no run yet");
-  if (ctx.last_report_off) return ngap_fail(NGAP_ERR_STATE, "last run had --report-off");
-  if (ctx.last_count > ctx.report_cap)
-    return ngap_fail(NGAP_ERR_OVERFLOW, "reports exceeded the capacity of the last run");
-  HIP_TRY(hipSetDevice(eng->device));
-  std::vector<uint64_t> v;
-  int rc = sorted_reports(ctx, v);
-  if (rc) return rc;
-  *n_out = v.size();
-  if (out) {
-    uint64_t n = std::min<uint64_t>(cap, v.size());
-    for (uint64_t i = 0; i < n; i++) out[i] = (v[i] << 32) | (v[i] >> 32);
-  }
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    RunCtx &ctx = eng->ctx;
+    if (!ctx.have_run) return ngap_fail(NGAP_ERR_STATE, "no run yet");
+    if (ctx.last_report_off) return ngap_fail(NGAP_ERR_STATE, "last run had --report-off");
+    if (ctx.last_count > ctx.report_cap)
+      return ngap_fail(NGAP_ERR_OVERFLOW, "reports exceeded the capacity of the last run");
+    HIP_TRY(hipSetDevice(eng->device));
+    std::vector<uint64_t> v;
+    int rc = sorted_reports(ctx, v);
+    if (rc) return rc;
+    *n_out = v.size();
+    if (out) {
+      uint64_t n = std::min<uint64_t>(cap, v.size());
+      for (uint64_t i = 0; i < n; i++) out[i] = (v[i] << 32) | (v[i] >> 32);
+    }
+    return NGAP_OK;
+  });
 }
 
 int ngap_get_task_items(ngap_engine *eng, uint64_t *out, uint64_t cap, uint64_t *n_out) {
-  if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  RunCtx &ctx = eng->ctx;
-  if (!ctx.have_run) return ngap_fail(NGAP_ERR_STATE, "no run yet");
-  if (!ctx.last_traces)
-    return ngap_fail(NGAP_ERR_STATE, "the last run did not record work-item traces (task_traces)");
-  *n_out = ctx.last_tasks.size();
-  if (out && !ctx.last_tasks.empty()) {
-    uint64_t n = std::min<uint64_t>(cap, ctx.last_tasks.size());
-    HIP_TRY(hipSetDevice(eng->device));
-    HIP_TRY(hipMemcpy(out, ctx.task_items.p, n * 8, hipMemcpyDeviceToHost));
-  }
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    RunCtx &ctx = eng->ctx;
+    if (!ctx.have_run) return ngap_fail(NGAP_ERR_STATE, "no run yet");
+    if (!ctx.last_traces)
+      return ngap_fail(NGAP_ERR_STATE, "the last run did not record work-item traces (task_traces)");
+    *n_out = ctx.last_tasks.size();
+    if (out && !ctx.last_tasks.empty()) {
+      uint64_t n = std::min<uint64_t>(cap, ctx.last_tasks.size());
+      HIP_TRY(hipSetDevice(eng->device));
+      HIP_TRY(hipMemcpy(out, ctx.task_items.p, n * 8, hipMemcpyDeviceToHost));
+    }
+    return NGAP_OK;
+  });
 }
 
 int ngap_get_tasks(ngap_engine *eng, ngap_task *out, uint64_t cap, uint64_t *n_out) {
-  if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  RunCtx &ctx = eng->ctx;
-  if (!ctx.have_run) return ngap_fail(NGAP_ERR_STATE, "no run yet");
-  *n_out = ctx.last_tasks.size();
-  const uint32_t L = (uint32_t)eng->chunk_len;
-  if (out)
-    for (uint64_t i = 0; i < std::min<uint64_t>(cap, ctx.last_tasks.size()); i++) {
-      const Task &t = ctx.last_tasks[i];
-      out[i] = {t.group, L ? t.cstart / L : 0, t.begin - t.cstart, t.end - t.cstart};
-    }
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    RunCtx &ctx = eng->ctx;
+    if (!ctx.have_run) return ngap_fail(NGAP_ERR_STATE, "no run yet");
+    *n_out = ctx.last_tasks.size();
+    const uint32_t L = (uint32_t)eng->chunk_len;
+    if (out)
+      for (uint64_t i = 0; i < std::min<uint64_t>(cap, ctx.last_tasks.size()); i++) {
+        const Task &t = ctx.last_tasks[i];
+        out[i] = {t.group, L ? t.cstart / L : 0, t.begin - t.cstart, t.end - t.cstart};
+      }
+    return NGAP_OK;
+  });
 }
 
 }  // extern "C"

@@ -17,6 +17,7 @@
 #include <unordered_map>
 
 #include "host_pool.h"
+#include "c_guard.h"
 #include "runtime.h"
 
 using namespace ngapamd;
@@ -430,154 +431,168 @@ extern "C" {
 
 int ngap_server_create(ngap_engine *eng, uint64_t budget, int32_t max_inflight,
                        const ngap_run_opts *opts, ngap_server **out) {
-  if (!eng || !out || !opts) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
-  if (max_inflight < 1 || max_inflight > 16)
-    return ngap_fail(NGAP_ERR_INVALID, "max_inflight must be in [1, 16]");
-  if (opts->report_off)
-    return ngap_fail(NGAP_ERR_INVALID,
-                     "report_off is not supported by the server: requests return their reports");
-  std::unique_ptr<ngap_server> s(new ngap_server());
-  s->eng = eng;
-  s->budget = budget ? budget : (200ull << 30);
-  s->opts = *opts;
-  std::string err;
-  if (!map_algorithm(opts->algorithm, opts->precompute_depth, eng->img.memo_depth, s->mp, err))
-    return ngap_fail(NGAP_ERR_INVALID, err);
-  HIP_TRY(hipSetDevice(eng->device));
-  uint32_t blocks = 0;
-  if (!ring_of(*opts))
-    return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
-  // request launches run the streaming kernel (chunk ends from the last-byte map)
-  int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts), 3);
-  if (rc) return rc;
-  // every launch gets the whole persistent grid: launches in flight share the
-  // CUs, and a lone launch is not held to a fraction of the GPU
-  s->blocks_per_launch = std::max<uint32_t>(1, blocks);
-  s->copier.reset(new HostPool());  // the copy of a 4 MiB request in up to 8 parts
-  for (int i = 0; i < max_inflight; i++) {
-    s->slots.emplace_back(new Slot());
-    rc = s->slots.back()->ctx.init();
+  return ngapamd::c_guard([&]() -> int {
+    if (!eng || !out || !opts) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (!eng->loaded) return ngap_fail(NGAP_ERR_STATE, "no groups loaded");
+    if (max_inflight < 1 || max_inflight > 16)
+      return ngap_fail(NGAP_ERR_INVALID, "max_inflight must be in [1, 16]");
+    if (opts->report_off)
+      return ngap_fail(NGAP_ERR_INVALID,
+                       "report_off is not supported by the server: requests return their reports");
+    std::unique_ptr<ngap_server> s(new ngap_server());
+    s->eng = eng;
+    s->budget = budget ? budget : (200ull << 30);
+    s->opts = *opts;
+    std::string err;
+    if (!map_algorithm(opts->algorithm, opts->precompute_depth, eng->img.memo_depth, s->mp, err))
+      return ngap_fail(NGAP_ERR_INVALID, err);
+    HIP_TRY(hipSetDevice(eng->device));
+    uint32_t blocks = 0;
+    if (!ring_of(*opts))
+      return ngap_fail(NGAP_ERR_INVALID, "ring_capacity must be a power of two in [64, 2048]");
+    // request launches run the streaming kernel (chunk ends from the last-byte map)
+    int rc = persistent_blocks(eng, opts->persistent_blocks, &blocks, ring_of(*opts), 3);
     if (rc) return rc;
-  }
-  *out = s.release();
-  return NGAP_OK;
+    // every launch gets the whole persistent grid: launches in flight share the
+    // CUs, and a lone launch is not held to a fraction of the GPU
+    s->blocks_per_launch = std::max<uint32_t>(1, blocks);
+    s->copier.reset(new HostPool());  // the copy of a 4 MiB request in up to 8 parts
+    for (int i = 0; i < max_inflight; i++) {
+      s->slots.emplace_back(new Slot());
+      rc = s->slots.back()->ctx.init();
+      if (rc) return rc;
+    }
+    *out = s.release();
+    return NGAP_OK;
+  });
 }
 
 int ngap_server_destroy(ngap_server *s) {
-  if (!s) return NGAP_OK;
-  for (auto &sl : s->slots)
-    if (sl->busy) (void)hipEventSynchronize(sl->ctx.ev1);
-  delete s;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!s) return NGAP_OK;
+    for (auto &sl : s->slots)
+      if (sl->busy) (void)hipEventSynchronize(sl->ctx.ev1);
+    delete s;
+    return NGAP_OK;
+  });
 }
 
 int ngap_server_submit(ngap_server *s, int32_t group, const uint8_t *data, uint64_t len,
                        uint64_t *req_id) {
-  if (!s || !req_id || (!data && len)) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  if (group < 0 || group >= s->eng->img.num_groups)
-    return ngap_fail(NGAP_ERR_INVALID, "group out of range");
-  if (len == 0 || len >= (1ull << 31)) return ngap_fail(NGAP_ERR_INVALID, "bad request length");
-  const uint64_t id = s->next_id++;
-  uint64_t cap = 0;
-  uint8_t *buf = s->pinned.get(len, &cap);
-  if (!buf) return ngap_fail(NGAP_ERR_OOM, "pinned host allocation failed");
-  s->copier->copy(buf, data, len);
-  Request &r = s->reqs[id];
-  r.group = group;
-  r.data = buf;
-  r.len = len;
-  r.cap = cap;
-  r.t_submit = now_ms();
-  s->queue.push_back(id);
-  s->queued_bytes += len;
-  *req_id = id;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!s || !req_id || (!data && len)) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    if (group < 0 || group >= s->eng->img.num_groups)
+      return ngap_fail(NGAP_ERR_INVALID, "group out of range");
+    if (len == 0 || len >= (1ull << 31)) return ngap_fail(NGAP_ERR_INVALID, "bad request length");
+    const uint64_t id = s->next_id++;
+    uint64_t cap = 0;
+    uint8_t *buf = s->pinned.get(len, &cap);
+    if (!buf) return ngap_fail(NGAP_ERR_OOM, "pinned host allocation failed");
+    s->copier->copy(buf, data, len);
+    Request &r = s->reqs[id];
+    r.group = group;
+    r.data = buf;
+    r.len = len;
+    r.cap = cap;
+    r.t_submit = now_ms();
+    s->queue.push_back(id);
+    s->queued_bytes += len;
+    *req_id = id;
+    return NGAP_OK;
+  });
 }
 
 int ngap_server_poll(ngap_server *s, int32_t block) {
-  if (!s) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  HIP_TRY(hipSetDevice(s->eng->device));
-  bool retired = false;
-  for (auto &sl : s->slots) {
-    if (!sl->busy) continue;
-    hipError_t q = hipEventQuery(sl->ctx.ev1);
-    if (q == hipSuccess) {
-      int rc = retire_slot(s, *sl);
-      if (rc) return rc;
-      retired = true;
-    } else if (q != hipErrorNotReady) {
-      HIP_TRY(q);
-    }
-  }
-  // an idle GPU takes the queue at once; while launches are in flight a free
-  // slot waits until kBusyLaunchBytes are queued or they finish, so that busy
-  // periods pack requests into fewer, larger launches (each launch costs host
-  // work and a kernel's fixed start-up, whatever its size)
-  bool busy_any = false;
-  for (auto &sl : s->slots) busy_any |= sl->busy;
-  for (auto &sl : s->slots) {
-    if (s->queue.empty()) break;
-    if (sl->busy) continue;
-    if (busy_any && s->queued_bytes < kBusyLaunchBytes) break;
-    int rc = launch_slot(s, *sl);
-    if (rc) return rc;
-    busy_any = true;
-  }
-  if (block && !retired) {
-    // wait for the oldest launch in flight
-    for (auto &sl : s->slots)
-      if (sl->busy) {
+  return ngapamd::c_guard([&]() -> int {
+    if (!s) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(s->eng->device));
+    bool retired = false;
+    for (auto &sl : s->slots) {
+      if (!sl->busy) continue;
+      hipError_t q = hipEventQuery(sl->ctx.ev1);
+      if (q == hipSuccess) {
         int rc = retire_slot(s, *sl);
         if (rc) return rc;
-        break;
+        retired = true;
+      } else if (q != hipErrorNotReady) {
+        HIP_TRY(q);
       }
-  }
-  return NGAP_OK;
+    }
+    // an idle GPU takes the queue at once; while launches are in flight a free
+    // slot waits until kBusyLaunchBytes are queued or they finish, so that busy
+    // periods pack requests into fewer, larger launches (each launch costs host
+    // work and a kernel's fixed start-up, whatever its size)
+    bool busy_any = false;
+    for (auto &sl : s->slots) busy_any |= sl->busy;
+    for (auto &sl : s->slots) {
+      if (s->queue.empty()) break;
+      if (sl->busy) continue;
+      if (busy_any && s->queued_bytes < kBusyLaunchBytes) break;
+      int rc = launch_slot(s, *sl);
+      if (rc) return rc;
+      busy_any = true;
+    }
+    if (block && !retired) {
+      // wait for the oldest launch in flight
+      for (auto &sl : s->slots)
+        if (sl->busy) {
+          int rc = retire_slot(s, *sl);
+          if (rc) return rc;
+          break;
+        }
+    }
+    return NGAP_OK;
+  });
 }
 
 int ngap_server_take(ngap_server *s, uint64_t req_id, int32_t *state, uint64_t *out,
                      uint64_t cap, uint64_t *n_out, double *latency_ms) {
-  if (!s || !state || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  auto it = s->reqs.find(req_id);
-  if (it == s->reqs.end()) return ngap_fail(NGAP_ERR_INVALID, "unknown request");
-  Request &r = it->second;
-  *state = r.state;
-  *n_out = r.reports.size();
-  if (latency_ms) *latency_ms = r.state >= 2 ? r.t_done - r.t_submit : -1.0;
-  if (r.state == 2 && out) {
-    const uint64_t n = std::min<uint64_t>(cap, r.reports.size());
-    std::copy(r.reports.begin(), r.reports.begin() + n, out);
-    if (n == r.reports.size()) s->reqs.erase(it);  // delivered
-  }
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!s || !state || !n_out) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return ngap_fail(NGAP_ERR_INVALID, "unknown request");
+    Request &r = it->second;
+    *state = r.state;
+    *n_out = r.reports.size();
+    if (latency_ms) *latency_ms = r.state >= 2 ? r.t_done - r.t_submit : -1.0;
+    if (r.state == 2 && out) {
+      const uint64_t n = std::min<uint64_t>(cap, r.reports.size());
+      std::copy(r.reports.begin(), r.reports.begin() + n, out);
+      if (n == r.reports.size()) s->reqs.erase(it);  // delivered
+    }
+    return NGAP_OK;
+  });
 }
 
 int ngap_server_pending(ngap_server *s, uint64_t *queued, uint64_t *inflight) {
-  if (!s || !queued || !inflight) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  *queued = s->queue.size();
-  uint64_t n = 0;
-  for (auto &sl : s->slots)
-    if (sl->busy) n += sl->ids.size();
-  *inflight = n;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!s || !queued || !inflight) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    *queued = s->queue.size();
+    uint64_t n = 0;
+    for (auto &sl : s->slots)
+      if (sl->busy) n += sl->ids.size();
+    *inflight = n;
+    return NGAP_OK;
+  });
 }
 
 int ngap_server_stats(ngap_server *s, ngap_server_info *info) {
-  if (!s || !info) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
-  memset(info, 0, sizeof *info);
-  info->budget_bytes = s->budget;
-  info->queued = s->queue.size();
-  for (auto &sl : s->slots)
-    if (sl->busy) info->inflight_requests += sl->ids.size();
-  info->inflight_bytes = s->inflight_bytes;
-  info->held_bytes = s->held();
-  info->peak_inflight_bytes = s->peak_inflight;
-  info->peak_held_bytes = s->peak_held;
-  info->launches = s->launches;
-  info->rejected = s->rejected;
-  info->redone = s->redone;
-  return NGAP_OK;
+  return ngapamd::c_guard([&]() -> int {
+    if (!s || !info) return ngap_fail(NGAP_ERR_INVALID, "NULL argument");
+    memset(info, 0, sizeof *info);
+    info->budget_bytes = s->budget;
+    info->queued = s->queue.size();
+    for (auto &sl : s->slots)
+      if (sl->busy) info->inflight_requests += sl->ids.size();
+    info->inflight_bytes = s->inflight_bytes;
+    info->held_bytes = s->held();
+    info->peak_inflight_bytes = s->peak_inflight;
+    info->peak_held_bytes = s->peak_held;
+    info->launches = s->launches;
+    info->rejected = s->rejected;
+    info->redone = s->redone;
+    return NGAP_OK;
+  });
 }
 
 }  // extern "C"
