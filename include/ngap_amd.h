@@ -340,7 +340,8 @@ int32_t ngap_frontend_num_groups(const ngap_frontend *fe);
 int ngap_frontend_group(const ngap_frontend *fe, int32_t g, ngap_group_desc *desc);
 /* Name of state s of group g (ANML id, with _dup_ suffixes after splits). */
 const char *ngap_frontend_state_name(const ngap_frontend *fe, int32_t g, int32_t s);
-/* Text dump of all groups (same format as the reference-side dumper). */
+/* Text dump of all groups (same format as the reference-side dumper); NULL
+ * when the text could not be built (ngap_last_error() says why). */
 const char *ngap_frontend_dump(ngap_frontend *fe);
 /* Parses one VASim symbol-set string into 8 words. */
 int ngap_parse_symbol_set(const char *s, uint32_t words_out[8]);

@@ -1,7 +1,9 @@
-// The body of every status-returning C entry point runs under c_guard: a C++
-// exception thrown inside (host memory exhausted, a thread the system
-// refused) becomes NGAP_ERR_HOST and a message for ngap_last_error(), so
-// nothing unwinds into a C caller.
+// The body of every C entry point runs under c_guard: a C++ exception thrown
+// inside (host memory exhausted, a thread the system refused) becomes
+// NGAP_ERR_HOST (or the entry point's own failure value, c_guard_or) and a
+// message for ngap_last_error(), so nothing unwinds into a C caller.  The
+// handlers allocate nothing they cannot do without: host memory exhausted is
+// reported through a message with static storage.
 #pragma once
 
 #include <exception>
@@ -10,22 +12,32 @@
 
 #include "../../include/ngap_amd.h"
 
-void ngap_set_last_error_internal(const std::string &m);  // errors.cpp
+void ngap_set_last_error_internal(const std::string &m) noexcept;  // errors.cpp
+void ngap_set_last_error_fixed(const char *m) noexcept;            // errors.cpp
 
 namespace ngapamd {
 
-template <class F>
-int c_guard(F &&body) {
+template <class F, class R>
+R c_guard_or(F &&body, R fail) noexcept {
   try {
     return body();
   } catch (const std::bad_alloc &) {
-    ngap_set_last_error_internal("host memory exhausted");
+    ngap_set_last_error_fixed("host memory exhausted");
   } catch (const std::exception &e) {
-    ngap_set_last_error_internal(std::string("host error: ") + e.what());
+    try {
+      ngap_set_last_error_internal(std::string("host error: ") + e.what());
+    } catch (...) {
+      ngap_set_last_error_fixed("host error");
+    }
   } catch (...) {
-    ngap_set_last_error_internal("host error");
+    ngap_set_last_error_fixed("host error");
   }
-  return NGAP_ERR_HOST;
+  return fail;
+}
+
+template <class F>
+int c_guard(F &&body) noexcept {
+  return c_guard_or(static_cast<F &&>(body), (int)NGAP_ERR_HOST);
 }
 
 }  // namespace ngapamd

@@ -16,7 +16,7 @@ struct ngap_frontend {
   std::string dump;
 };
 
-void ngap_set_last_error_internal(const std::string &m);  // errors.cpp
+void ngap_set_last_error_internal(const std::string &m) noexcept;  // errors.cpp
 
 namespace {
 int fe_fail(const std::string &m) {
@@ -101,9 +101,11 @@ const char *ngap_frontend_state_name(const ngap_frontend *fe, int32_t g, int32_t
 }
 
 const char *ngap_frontend_dump(ngap_frontend *fe) {
-  if (!fe) return nullptr;
-  fe->dump = dump_groups(fe->groups);
-  return fe->dump.c_str();
+  return ngapamd::c_guard_or([&]() -> const char * {
+    if (!fe) return nullptr;
+    fe->dump = dump_groups(fe->groups);
+    return fe->dump.c_str();
+  }, (const char *)nullptr);
 }
 
 int ngap_parse_symbol_set(const char *s, uint32_t words[8]) {

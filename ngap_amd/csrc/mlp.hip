@@ -19,7 +19,7 @@
 #include "../../include/ngap_amd.h"
 #include "c_guard.h"
 
-void ngap_set_last_error_internal(const std::string &m);
+void ngap_set_last_error_internal(const std::string &m) noexcept;
 
 namespace {
 

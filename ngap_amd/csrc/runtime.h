@@ -14,7 +14,7 @@
 #include "kernels.h"
 #include "layout.h"
 
-void ngap_set_last_error_internal(const std::string &m);  // errors.cpp
+void ngap_set_last_error_internal(const std::string &m) noexcept;  // errors.cpp
 int ngap_fail(int code, const std::string &msg);
 
 #define HIP_TRY(expr)                                                                 \

@@ -29,6 +29,18 @@ int main() {
          "system_error");
   expect(std::strstr(ngap_last_error(), "host error") != nullptr, "system_error message");
   expect(c_guard([]() -> int { throw 7; }) == NGAP_ERR_HOST, "anything else");
+  expect(std::strcmp(ngap_last_error(), "host error") == 0, "anything else message");
+  // entry points that return a pointer: NULL on an exception
+  static const char text[] = "text";
+  expect(ngapamd::c_guard_or([]() -> const char * { return text; }, (const char *)nullptr) == text,
+         "pointer passes");
+  expect(ngapamd::c_guard_or([]() -> const char * { throw std::bad_alloc(); },
+                             (const char *)nullptr) == nullptr,
+         "pointer on bad_alloc");
+  expect(std::strcmp(ngap_last_error(), "host memory exhausted") == 0, "pointer message");
+  // a stored message replaces the static one
+  ngap_set_last_error_internal("worklist spill capacity exceeded");
+  expect(std::strcmp(ngap_last_error(), "worklist spill capacity exceeded") == 0, "stored message");
   if (!fails) std::printf("PASS\n");
   return fails ? 1 : 0;
 }
