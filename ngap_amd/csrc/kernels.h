@@ -78,7 +78,7 @@ struct KParams {
   uint32_t spill_cap;      // items per wave
   uint32_t *overflow;      // plain: per unit; batched: per task
   uint32_t *overflow_count;  // units / tasks that set their overflow flag
-  unsigned long long *items;   // expanded items (statistic)
+  unsigned long long *items;   // expanded items (statistic): kItemLanes lanes
   const Task *tasks;        // nullable: work items generated from their index
   // Generated work items (tasks == nullptr): chunks cut into windows of
   // gen_window positions; all-group work items (gen_cross) ordered by
@@ -109,6 +109,22 @@ constexpr uint32_t kMaxOpen = 8;  // work items a wave overlaps (k_batched)
 // 128-byte line.  A wave whose queue is empty takes from the next ones.
 constexpr uint32_t kQueues = 8, kQueueStride = 32;
 constexpr uint32_t kBlockThreads = 256;
+// Counters of a run (one buffer per run context, cleared by k_reset): bytes
+// 0-31 hold the report count and the count of overflowed work items (word
+// 3), byte 128 on the queue claim counters (one line each), then kItemLanes
+// lines of expanded-item counts, one 64-bit word at the start of each line:
+// the waves of workgroup b add theirs at lane b % kItemLanes and the host
+// sums the lanes.  A single count that every wave adds to as it ends costs
+// 6-8 % of a launch of the streaming kernel: atomics on one address
+// serialise.
+constexpr uint32_t kItemLanes = 32, kItemStride = 16;  // stride in 64-bit words
+constexpr uint32_t kItemsWord = (128 + kQueues * kQueueStride * 4) / 8;
+constexpr uint32_t kCounterBytes = (kItemsWord + kItemLanes * kItemStride) * 8;
+NGAP_HD inline unsigned long long items_sum(const unsigned long long *counters) {
+  unsigned long long n = 0;
+  for (uint32_t k = 0; k < kItemLanes; k++) n += counters[kItemsWord + k * kItemStride];
+  return n;
+}
 
 // Work item t of a generated schedule (KParams::gen_window).
 NGAP_HD inline Task gen_task(const KParams &P, uint32_t t) {

@@ -66,6 +66,11 @@ NGAP_DEV uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
   return x;
 }
 
+// A wave's expanded items, added at its workgroup's lane of the count.
+NGAP_DEV void add_items(const KParams &P, unsigned long long n) {
+  atomicAdd(P.items + (blockIdx.x % kItemLanes) * kItemStride, n);
+}
+
 struct WaveState {
   uint64_t *ring;
   uint64_t *rbuf;
@@ -1025,7 +1030,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_plain(KParams P) {
   flush_reports(P, W);
   if (W.lane == 0) {
     if (W.overflow && atomicOr(&P.overflow[unit], 1u) == 0u) atomicAdd(P.overflow_count, 1u);
-    atomicAdd(P.items, W.items);
+    add_items(P, W.items);
   }
 }
 
@@ -1066,7 +1071,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched(KParams P) {
     }
   }
   flush_reports(P, W);
-  if (W.lane == 0) atomicAdd(P.items, W.items);
+  if (W.lane == 0) add_items(P, W.items);
 }
 
 // The batched engine without per-work-item traces: each wave overlaps its
@@ -1085,7 +1090,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_batched_stream(KParams P) {
   clear_dedup(P, W);
   drain_stream<false, End, Gen>(P, W);
   flush_reports(P, W);
-  if (W.lane == 0) atomicAdd(P.items, W.items);
+  if (W.lane == 0) add_items(P, W.items);
 }
 
 // The streaming kernel after a root pass (k_roots): its work items' roots
@@ -1100,7 +1105,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_expand(KParams P) {
   clear_dedup(P, W);
   drain_stream<true, kEndChunkLen, false>(P, W);
   flush_reports(P, W);
-  if (W.lane == 0) atomicAdd(P.items, W.items);
+  if (W.lane == 0) add_items(P, W.items);
 }
 
 // Root pass of an engine run: one wave per piece of a work item (a non-
@@ -1254,7 +1259,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_blocking(KParams P) {
   flush_reports(P, W);
   if (W.lane == 0) {
     if (W.overflow && atomicOr(&P.overflow[unit], 1u) == 0u) atomicAdd(P.overflow_count, 1u);
-    atomicAdd(P.items, W.items);
+    add_items(P, W.items);
   }
 }
 
@@ -1267,7 +1272,7 @@ __global__ void k_reset(unsigned long long *report_count, unsigned long long *it
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     if (!keep_reports) *report_count = 0;
-    *items = 0;
+    for (uint32_t k = 0; k < kItemLanes; k++) items[k * kItemStride] = 0;
     for (uint32_t k = 0; k < kQueues; k++) task_head[k * kQueueStride] = 0;
     *overflow_count = 0;
   }

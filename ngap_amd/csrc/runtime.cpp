@@ -65,19 +65,20 @@ int RunCtx::init() {
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
-  HIP_TRY(counters.reserve(128 + kQueues * kQueueStride * 4));
+  HIP_TRY(counters.reserve(kCounterBytes));
   // the reset kernel clears the counters a run uses; the rest stays zero
   HIP_TRY(hipMemset(counters.p, 0, counters.n));
-  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&host_cnt), 32, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&host_cnt), kCounterBytes, hipHostMallocDefault));
   return NGAP_OK;
 }
 
 int RunCtx::finish(float *ms, unsigned long long cnt[4], bool record_end) {
   if (record_end) HIP_TRY(hipEventRecord(ev1, stream));
-  HIP_TRY(hipMemcpyAsync(host_cnt, counters.p, 32, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipMemcpyAsync(host_cnt, counters.p, kCounterBytes, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   HIP_TRY(hipEventElapsedTime(ms, ev0, ev1));
   memcpy(cnt, host_cnt, 32);
+  cnt[1] = items_sum(host_cnt);
   return NGAP_OK;
 }
 
@@ -188,7 +189,7 @@ KParams base_params(const ngap_engine *eng, const Mapping &mp, const ngap_run_op
 void attach_ctx(KParams &P, RunCtx &ctx) {
   unsigned long long *counters = ctx.counters.as<unsigned long long>();
   P.report_count = counters + 0;
-  P.items = counters + 1;
+  P.items = counters + kItemsWord;
   P.task_head = reinterpret_cast<uint32_t *>(counters + 16);  // byte 128: the queue counters
   P.overflow_count = reinterpret_cast<uint32_t *>(counters + 3);
   P.reports = ctx.reports.as<uint64_t>();

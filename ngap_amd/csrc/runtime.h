@@ -121,9 +121,9 @@ struct RunCtx {
   int init();
   void drop_graph();
   // Ends a timed launch begun with ev0: records ev1 (unless the launch
-  // recorded it), queues the copy of the four run counters (reports, items,
-  // -, overflowed units) into host_cnt behind it and waits for the stream
-  // once.  *ms = ev0..ev1.
+  // recorded it), queues the copy of the run counters into host_cnt behind
+  // it and waits for the stream once.  *ms = ev0..ev1; cnt = reports, items
+  // (the lanes summed), -, overflowed units.
   int finish(float *ms, unsigned long long cnt[4], bool record_end);
 };
 
