@@ -111,14 +111,16 @@ constexpr uint32_t kQueues = 8, kQueueStride = 32;
 constexpr uint32_t kBlockThreads = 256;
 // Counters of a run (one buffer per run context, cleared by k_reset): bytes
 // 0-31 hold the report count and the count of overflowed work items (word
-// 3), byte 128 on the queue claim counters (one line each), then kItemLanes
-// lines of expanded-item counts, one 64-bit word at the start of each line:
+// 3), byte 128 on the queue claim counters, then one mask per XCD of the
+// queues its waves found empty (one line each), then kItemLanes lines of
+// expanded-item counts, one 64-bit word at the start of each line:
 // the waves of workgroup b add theirs at lane b % kItemLanes and the host
 // sums the lanes.  A single count that every wave adds to as it ends costs
 // 6-8 % of a launch of the streaming kernel: atomics on one address
 // serialise.
 constexpr uint32_t kItemLanes = 32, kItemStride = 16;  // stride in 64-bit words
-constexpr uint32_t kItemsWord = (128 + kQueues * kQueueStride * 4) / 8;
+constexpr uint32_t kEmptyMasks = kQueues * kQueueStride;  // task_head word of XCD 0's mask
+constexpr uint32_t kItemsWord = (128 + 2 * kQueues * kQueueStride * 4) / 8;
 constexpr uint32_t kCounterBytes = (kItemsWord + kItemLanes * kItemStride) * 8;
 NGAP_HD inline unsigned long long items_sum(const unsigned long long *counters) {
   unsigned long long n = 0;

@@ -821,16 +821,29 @@ NGAP_DEV void drain(const KParams &P, WaveState &W, Src &src) {
 }
 
 // Claims the next work item for this wave: from the queue of its XCD first,
-// then from the others in turn.  Returns P.num_tasks when all are empty.
-// `q` is the wave's current queue (advanced when one runs dry).
+// then from the others.  Returns P.num_tasks when all are empty.  `q` is the
+// wave's current queue (advanced when one runs dry).  A wave that finds a
+// queue empty adds it to its XCD's mask of empty queues and gets back what
+// the XCD's other waves found, so it claims only from queues not known to be
+// empty (a queue found empty stays empty): at the end of a run a wave learns
+// that every queue is empty from two atomics, not from a failed claim on each
+// of the eight counters, which thousands of waves would hit at once.
 NGAP_DEV uint32_t claim_task(const KParams &P, const WaveState &W, uint32_t &q) {
+  uint32_t *empty = P.task_head + kEmptyMasks + (blockIdx.x % kQueues) * kQueueStride;
+  uint32_t known = 0;
   for (uint32_t tries = 0; tries < kQueues; tries++) {
     uint32_t j = 0;
     if (W.lane == 0) j = atomicAdd(P.task_head + q * kQueueStride, 1u);
     j = __shfl(j, 0, 64);
     const uint64_t t = (uint64_t)j * kQueues + q;
     if (t < P.num_tasks) return (uint32_t)t;
-    q = (q + 1) % kQueues;
+    uint32_t m = 0;
+    if (W.lane == 0) m = atomicOr(empty, 1u << q);
+    known |= __shfl(m, 0, 64) | (1u << q);
+    if (known == (1u << kQueues) - 1) return P.num_tasks;
+    uint32_t d = 1;
+    while ((known >> ((q + d) % kQueues)) & 1u) d++;
+    q = (q + d) % kQueues;
   }
   return P.num_tasks;
 }
@@ -1273,7 +1286,7 @@ __global__ void k_reset(unsigned long long *report_count, unsigned long long *it
   if (i == 0) {
     if (!keep_reports) *report_count = 0;
     for (uint32_t k = 0; k < kItemLanes; k++) items[k * kItemStride] = 0;
-    for (uint32_t k = 0; k < kQueues; k++) task_head[k * kQueueStride] = 0;
+    for (uint32_t k = 0; k < 2 * kQueues; k++) task_head[k * kQueueStride] = 0;
     *overflow_count = 0;
   }
   for (; i < n_overflow; i += gridDim.x * blockDim.x) {
