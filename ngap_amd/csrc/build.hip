@@ -595,7 +595,8 @@ int build_image_device(ngap_engine *eng, const HostGroup *hg, int32_t G, int32_t
       err = "CSR row offsets are not monotone";
       return NGAP_ERR_INVALID;
     }
-    if (E > 0 && !hg[g].col) {
+    if ((E > 0 && !hg[g].col) || (hg[g].num_aas > 0 && !hg[g].aas) ||
+        (hg[g].num_sod > 0 && !hg[g].sod)) {
       err = "group arrays missing";
       return NGAP_ERR_INVALID;
     }

@@ -93,6 +93,9 @@ bool build_part(const HostGroup &hg, int32_t g, int memo_depth, GroupPart &P) {
   }
   if (n > 0 && (!hg.row || !hg.sets || !hg.attrs)) return fail_part(P, "group arrays missing");
   const int E = n > 0 ? hg.row[n] : 0;
+  if (E < 0) return fail_part(P, "CSR row offsets are not monotone");
+  if ((E > 0 && !hg.col) || (hg.num_aas > 0 && !hg.aas) || (hg.num_sod > 0 && !hg.sod))
+    return fail_part(P, "group arrays missing");
   for (int v = 0; v < n; v++)
     if (hg.row[v] > hg.row[v + 1] || hg.row[v] < 0)
       return fail_part(P, "CSR row offsets are not monotone");
