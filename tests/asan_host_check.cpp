@@ -1,15 +1,26 @@
-// The host-side front-end (ANML -> groups) and image builder compiled with
-// AddressSanitizer and UBSan on the CPU: every ANML file named on the command
-// line goes through the whole pipeline (two group counts) and the image
-// builder at precompute depths 1-3; then malformed groups, each of which the
-// builder must reject with a message and without touching memory it does not
-// own.  Prints PASS or the first failures.
+// The host-side front-end (ANML -> groups), image builder and the CLI's
+// --validation check compiled with AddressSanitizer and UBSan on the CPU:
+// every ANML file named on the command line goes through the whole pipeline
+// (two group counts) and the image builder at precompute depths 1-3, and the
+// CLI's CPU check (cli_validate.cpp) must give the oracle's reports on seeded
+// inputs over the automata's alphabet (one chunk and eight, with and without
+// duplicated chunks); then malformed groups, each of which the builder must
+// reject with a message and without touching memory it does not own.  Prints
+// PASS or the first failures.
 #include <cstdio>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "../ngap_amd/csrc/frontend.h"
 #include "../ngap_amd/csrc/layout.h"
+// the oracle, the checker, compiled into this test alone
+#include "../oracle/oracle.cpp"
+
+namespace ngapamd {
+std::vector<uint64_t> cpu_reference(const std::vector<GroupGraph> &gs, const uint8_t *input,
+                                    uint64_t chunk_len, uint32_t num_seg, bool is_dup);
+}
 
 using namespace ngapamd;
 
@@ -33,6 +44,46 @@ static HostGroup host_of(const GroupGraph &g) {
   return h;
 }
 
+// The CLI's CPU check against the oracle on seeded inputs over the bytes the
+// automata's classes hold (so that chains advance).
+static void check_validator(const char *path, const std::vector<GroupGraph> &groups) {
+  std::vector<int> alphabet;
+  for (int b = 0; b < 256; b++) {
+    bool any = false;
+    for (const GroupGraph &g : groups)
+      for (int v = 0; v < g.num_states && !any; v++)
+        any = (g.sets[(size_t)v * 8 + b / 32] >> (b % 32)) & 1u;
+    if (any) alphabet.push_back(b);
+  }
+  if (alphabet.empty()) return;
+  std::mt19937 rng(1234);
+  const uint32_t n = 4096;
+  std::vector<uint8_t> input(n);
+  for (auto &x : input)
+    x = rng() % 8 == 0 ? (uint8_t)(rng() % 256) : (uint8_t)alphabet[rng() % alphabet.size()];
+  std::vector<oracle_group> og;
+  for (const GroupGraph &g : groups)
+    og.push_back({g.num_states, g.row.data(), g.col.data(), g.sets.data(), g.attrs.data(),
+                  (int32_t)g.aas.size(), g.aas.data(), (int32_t)g.sod.size(), g.sod.data()});
+  for (uint32_t chunk : {n, n / 8})
+    for (int dup : {0, 1}) {
+      const std::vector<uint64_t> got = cpu_reference(groups, input.data(), chunk, n / chunk, dup);
+      std::vector<uint64_t> want(got.size() + 4096);
+      uint64_t nw = 0;
+      oracle_stats st;
+      if (oracle_run((int32_t)og.size(), og.data(), input.data(), chunk, n / chunk, dup, nullptr,
+                     1, 1, want.data(), want.size(), &nw, &st) != 0) {
+        fail(std::string(path) + ": oracle output larger than expected");
+        continue;
+      }
+      want.resize(nw);
+      if (got != want)
+        fail(std::string(path) + ": CLI validator differs from the oracle (chunk " +
+             std::to_string(chunk) + ", dup " + std::to_string(dup) + ": " +
+             std::to_string(got.size()) + " vs " + std::to_string(want.size()) + " reports)");
+    }
+}
+
 static void check_file(const char *path) {
   Nfa nfa;
   std::string err;
@@ -44,6 +95,7 @@ static void check_file(const char *path) {
     FrontendStats st;
     if (!build_groups(nfa, opt, groups, st, err)) return fail(std::string(path) + ": " + err);
     if (dump_groups(groups).empty()) fail(std::string(path) + ": empty dump");
+    check_validator(path, groups);
     std::vector<HostGroup> hg;
     for (const GroupGraph &g : groups) hg.push_back(host_of(g));
     for (int depth = 1; depth <= 3; depth++) {

@@ -1,7 +1,8 @@
-"""The host-side front-end and image builder under AddressSanitizer and UBSan
-(tests/asan_host_check.cpp, compiled with g++ on the CPU): every ANML fixture
-through the pipeline and the builder at depths 1-3, and malformed groups that
-must be rejected."""
+"""The host-side front-end, image builder and the CLI's --validation check
+under AddressSanitizer and UBSan (tests/asan_host_check.cpp, compiled with g++
+on the CPU): every ANML fixture through the pipeline and the builder at depths
+1-3, the CLI's check against the oracle on seeded inputs, and malformed groups
+that must be rejected."""
 import glob
 import os
 import shutil
@@ -20,7 +21,8 @@ def test_frontend_and_builder_under_asan(tmp_path):
                     "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
                     "-pthread", "-o", str(exe),
                     os.path.join(ROOT, "tests", "asan_host_check.cpp"),
-                    os.path.join(src, "frontend.cpp"), os.path.join(src, "layout.cpp")],
+                    os.path.join(src, "frontend.cpp"), os.path.join(src, "layout.cpp"),
+                    os.path.join(src, "cli_validate.cpp"), "-fopenmp"],
                    check=True)
     anml = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "*.anml")) +
                   glob.glob(os.path.join(ROOT, "tests", "golden", "frontend", "*.anml")))
