@@ -51,6 +51,27 @@ def test_null_arguments_are_errors():
     assert L.ngap_frontend_free(None) == _lib.NGAP_OK
 
 
+def test_every_entry_point_rejects_null_handles():
+    """Each status-returning entry point, called with NULL handles and
+    pointers (and zero sizes), returns NGAP_ERR_INVALID with a message before
+    it touches a device; the destroy/free calls accept NULL like free()."""
+    text = open(os.path.join(ROOT, "include", "ngap_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\bint\s+(ngap_[a-z_0-9]+)\s*\(", text)))
+    assert len(names) >= 30
+    raw = C.CDLL(_lib.LIB_PATH)   # no argtypes: every argument is passed as 0 / NULL
+    raw.ngap_last_error.restype = C.c_char_p
+    for n in names:
+        f = getattr(raw, n)
+        f.restype = C.c_int
+        rc = f(*([None] * 10))
+        if n.endswith(("_destroy", "_free")):
+            assert rc == _lib.NGAP_OK, n
+        else:
+            assert rc == _lib.NGAP_ERR_INVALID, (n, rc)
+            assert raw.ngap_last_error(), n
+
+
 def test_ctypes_structs_match_the_header_layout(tmp_path):
     """Size and every field offset of the ctypes mirrors equal what a C
     compiler lays out from include/ngap_amd.h."""
