@@ -96,3 +96,33 @@ def test_ctypes_structs_match_the_header_layout(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+_OOM_SCRIPT = r"""
+import ctypes as C, os, resource, sys
+L = C.CDLL(sys.argv[1])
+L.ngap_last_error.restype = C.c_char_p
+n = 200000
+text = ('<anml version="1.0"><automata-network id="x">' + "".join(
+    f'<state-transition-element id="s{i}" symbol-set="[ab]" start="all-input">'
+    f'<activate-on-match element="s{(i + 1) % n}"/></state-transition-element>'
+    for i in range(n)) + '</automata-network></anml>').encode()
+vm = int(open("/proc/self/statm").read().split()[0]) * os.sysconf("SC_PAGE_SIZE")
+resource.setrlimit(resource.RLIMIT_AS, (vm + (64 << 20), resource.getrlimit(resource.RLIMIT_AS)[1]))
+h = C.c_void_p()
+rc = L.ngap_frontend_load_text(text, 1, -1, C.byref(h))
+print(rc, L.ngap_last_error().decode())
+"""
+
+
+def test_host_memory_exhaustion_is_a_status():
+    """A C++ allocation failure inside an entry point (the front-end parsing a
+    200,000-state network under a 64 MiB address-space margin) comes back as
+    NGAP_ERR_HOST with a message, not as an exception through the C ABI."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, "-c", _OOM_SCRIPT, _lib.LIB_PATH],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    rc, msg = out.stdout.strip().split(" ", 1)
+    assert int(rc) == _lib.NGAP_ERR_HOST and "memory" in msg, out.stdout
