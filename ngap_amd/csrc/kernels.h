@@ -116,7 +116,7 @@ constexpr uint32_t kBlockThreads = 256;
 // expanded-item counts, one 64-bit word at the start of each line:
 // the waves of workgroup b add theirs at lane b % kItemLanes and the host
 // sums the lanes.  A single count that every wave adds to as it ends costs
-// 6-8 % of a launch of the streaming kernel: atomics on one address
+// 5-8 % of a launch of the streaming kernel: atomics on one address
 // serialise.
 constexpr uint32_t kItemLanes = 32, kItemStride = 16;  // stride in 64-bit words
 constexpr uint32_t kEmptyMasks = kQueues * kQueueStride;  // task_head word of XCD 0's mask
